@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark: Toeplitz A^H A applies/s on one MI355X.
+
+Default workload (one step = one A^H A apply, inputs resident in HBM):
+stack-of-stars 320x320x56 image, 640x640x56 Toeplitz grid, L=16, C=12,
+400 golden-angle spokes x 640 samples on every kz plane, frames round-robin
+over 200, synthetic data, kernel built once with mr_calc_toep.
+
+Prints one JSON line. `roofline` is for the dominant kernel of the apply:
+its algorithmic bytes per launch divided by its average launch time, the
+latter measured with HIP events on the library's stream over the timed
+region (MR_AHA_HIP_PROFILE). `cpu_baseline` times the oracle's CPU Toeplitz
+apply (complex64, pocketfft threads) on one of the C coils of the same
+workload and scales to applies/s.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_bytes(cfg, nc):
+    """Algorithmic HBM bytes of one launch of each stage for a group of nc coils."""
+    Nx, Ny, Nz = cfg.n_dim
+    Mx, My, Mz = cfg.phi2_dim
+    L = cfg.L
+    V = L * nc
+    T = L * (L + 1) // 2
+    N = Nx * Ny * Nz
+    P = Mx * My * Mz
+    c8 = 8
+    three = cfg.n_spatial == 3
+    rows_x = Mx * Ny * Nz * V * c8      # x-transformed image rows
+    rows_xy = Mx * My * Nz * V * c8     # x,y-transformed planes
+    full = P * V * c8
+    b = {
+        "F1": N * L * c8 + N * nc * c8 + rows_x,
+        "F2": rows_x + rows_xy,
+        "F3": (rows_xy + full) if three else 0,
+        "BD": 2 * full + T * P * c8,
+        "I3": (full + rows_xy) if three else 0,
+        "I2": rows_xy + rows_x,
+        "I1": rows_x + N * nc * c8 + N * L * c8,
+    }
+    return b
+
+
+def apply_model_bytes(cfg):
+    """Stage-model algorithmic bytes of one apply: T P 8 + 4 C L P 8 + (2L + 2C) N 8."""
+    Nx, Ny, Nz = cfg.n_dim
+    Mx, My, Mz = cfg.phi2_dim
+    L, C = cfg.L, cfg.n_coils
+    T = L * (L + 1) // 2
+    N = Nx * Ny * Nz
+    P = Mx * My * Mz
+    return T * P * 8 + 4 * C * L * P * 8 + (2 * L + 2 * C) * N * 8
+
+
+def cpu_baseline(w, K, cfg, budget_s=30.0):
+    """Oracle CPU Toeplitz apply on one coil of the workload (complex64)."""
+    import numpy as np
+    from oracle import cpu_baseline as cb
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(int(threads), 16))
+    x = w["x"].cpu().numpy()
+    S = w["SEs"][:1].cpu().numpy()
+    Kh = K.cpu().numpy()
+    n = list(cfg.n_dim)
+    M = list(cfg.phi2_dim)
+    t0 = time.perf_counter()
+    cb.toeplitz_apply_coils(n, M, cfg.L, x, S, Kh, cfg.triangulation == 1, [0], threads)
+    dt = time.perf_counter() - t0
+    del Kh
+    C = cfg.n_coils
+    return {"value": 1.0 / (dt * C), "unit": "A^H A applies/s", "cores": threads, "kind": "port",
+            "sample": f"1 of {C} coils of one apply (all {cfg.L} ranks, full grid), "
+                      f"{dt:.1f} s, scaled x{C}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="stack", choices=["stack", "radial", "koosh"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print("bench.py measures one GPU: the multi-GPU coil split is not part of this build "
+              "(every run here is single-GPU); rerun with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import mr_aha_amd as mr
+    from mr_aha_amd import _lib, workloads
+
+    torch.cuda.set_device(0)
+    dev = "cuda:0"
+    t_gen = time.perf_counter()
+    if args.config == "stack":
+        w = workloads.stack_of_stars(device=dev)
+        workload = "stack-of-stars 320x320x56, grid 640x640x56, L=16, C=12, 400x640 spokes"
+    elif args.config == "radial":
+        w = workloads.radial_2d(device=dev)
+        workload = "2D radial 128x128, grid 256x256, L=4, C=4, 64x256 spokes"
+    else:
+        w = workloads.kooshball(device=dev)
+        workload = "3D kooshball 160^3, grid 320^3, L=8, C=8, 20000x320 spokes"
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t_gen
+    cfg = mr.make_config(w["traj_type"], w["n_dim"], w["L"], n_coils=w["n_coils"],
+                         mem_fraction=0.9)
+    print(f"[bench] {workload}: inputs generated in {t_gen:.1f} s", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    op = mr.create_op(cfg, w["SEs"], w["phi2"], w["traj"])
+    torch.cuda.synchronize()
+    t_create = time.perf_counter() - t0
+    info = op.info
+    print(f"[bench] operator created in {t_create:.1f} s (kernel {info['toep_ms'] / 1e3:.1f} s), "
+          f"group {info['group'][0]}", file=sys.stderr, flush=True)
+
+    x = w["x"]
+    out = torch.empty_like(x)
+    for _ in range(args.warmup):
+        op.apply(x, out)
+    torch.cuda.synchronize()
+
+    os.environ["MR_AHA_HIP_PROFILE"] = "1"
+    ms = (ctypes.c_double * 8)()
+    calls = (ctypes.c_int * 8)()
+    _lib.check(_lib.lib.mr_op_profile(op._handle, ms, calls, 8), "mr_op_profile")  # reset
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    t_wall = time.perf_counter()
+    for _ in range(args.steps):
+        op.apply(x, out)
+    e1.record()
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    os.environ.pop("MR_AHA_HIP_PROFILE")
+    _lib.check(_lib.lib.mr_op_profile(op._handle, ms, calls, 8), "mr_op_profile")
+    gpu_ms = e0.elapsed_time(e1)
+    ms_per_step = gpu_ms / args.steps
+    value = 1000.0 / ms_per_step
+
+    group = info["group"][0]
+    sb = stage_bytes(cfg, group)
+    stages = {}
+    for i, name in enumerate(_lib.PROFILE_STAGES):
+        if calls[i]:
+            avg = ms[i] / calls[i]
+            nb = sb.get(name, 0)
+            stages[name] = {"avg_ms": round(avg, 4), "launches": int(calls[i]),
+                            "gbs": round(nb / (avg * 1e6), 1) if nb else None}
+    dom = max((s for s in stages if s in sb), key=lambda s: stages[s]["avg_ms"])
+    dom_ms = stages[dom]["avg_ms"]
+    achieved = sb[dom] / (dom_ms * 1e6)
+    B = apply_model_bytes(cfg)
+
+    cpu = None
+    if not args.no_cpu:
+        K = op.kernel(0)
+        cpu = cpu_baseline(w, K, cfg)
+        del K
+    op.close()
+
+    line = {
+        "metric": "A^H A applies/s @320x320x56 L16 C12 (mixed vs FP32); p50/p99 recon latency",
+        "value": round(value, 3), "unit": "A^H A applies/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 (fp32 complex)", "data": "synthetic",
+        "config": {"workload": workload, "precision": "fp32 kernel", "coil_group": group,
+                   "kernel_build_s": round(info["toep_ms"] / 1e3, 2)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "algorithmic_bytes_per_launch": sb[dom]},
+        "apply_model": {"bytes": B, "gbs": round(B / (ms_per_step * 1e6), 1),
+                        "frac": round(B / (ms_per_step * 1e6) / HBM_PEAK_GBS, 4)},
+        "stages": stages,
+        "wall_ms_per_step": round(t_wall * 1000 / args.steps, 3),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,31 @@
+/*
+ * mr_aha_ext.h — entry points of libmr_aha_c that have no counterpart in the
+ * reference's C ABI. They serve measurement and the MI355X-specific
+ * features; the drop-in ABI is mr_aha.h alone.
+ */
+#ifndef MR_AHA_MR_AHA_EXT_H
+#define MR_AHA_MR_AHA_EXT_H
+
+#include "mr_aha/mr_aha.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Stages timed when MR_AHA_HIP_PROFILE=1 is set at execute time. */
+#define MR_PROFILE_STAGES 8  /* F1 F2 F3 BD I3 I2 I1 other */
+
+/* GPU milliseconds and launch counts per stage accumulated by the plan's
+ * executions since the previous call, then reset. Synchronises with the
+ * last recorded stage. No counterpart in the reference (its
+ * MR_AHA_CUDA_PROFILE prints totals to stderr, src/cuda/mr_aha.cu:516-623);
+ * for benchmarks that need per-kernel durations. */
+MR_EXPORT int mr_aha_plan_profile(mr_aha_plan plan, double *ms, int *calls, int n);
+/* Same for the plan inside a Toeplitz operator. */
+MR_EXPORT int mr_op_profile(mr_op op, double *ms, int *calls, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,651 @@
+// Toeplitz A^H A apply on MI355X: plan life cycle and the execute pipeline.
+//
+// Per coil group (G coils, V = L*G rank volumes), the operator
+//   AhAx = sum_c S_c^H P^H F^-1 T F P S_c x           (mr_aha.h of the reference, :580-603)
+// runs as fused axis passes of the project FFT engine and one block pass:
+//   F1  x-FFT of every image row, loading S_c * x_l zero-padded in natural
+//       order (pad and sensitivity fused into the load)    -> A1 [Mx, Ny, Nz, V]
+//   F2  y-FFT, zero-padding the Ny rows on load            -> A2 [Mx, My, Nz, V]
+//   F3  z-FFT (3D / stack), zero-padding the Nz planes    -> K  [Mx, My, Mz, V]
+//   BD  per-location Hermitian block multiply v <- v B(k), in place on K
+//   I3  inverse z-FFT, keeping the Nz image planes        -> A2'
+//   I2  inverse y-FFT, keeping the Ny image rows          -> A1'
+//   I1  inverse x-FFT, crop, conj(S_c) and the coil sum fused into the store
+// Only the image region is ever loaded or stored outside k-space: the zero
+// rows of the padding are synthesised in LDS and the discarded rows of the
+// inverse transforms are never written.
+//
+// Reference behaviour followed: stage order and coil grouping of
+// src/cuda/mr_aha.cu:694-790; index maps of src/cuda/mr_kernels.cu:98-176;
+// the block walk of src/cuda/mr_kernels_bd.cu:74-139 (each stored element
+// read once and applied to both of its mirror positions).
+
+#include "mr_hip.hpp"
+#include "mr_aha/mr_aha_ext.h"
+
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+// ---------------------------------------------------------------------------
+// Fused pass functors
+// ---------------------------------------------------------------------------
+
+// F1 load: pencil p = yz + NyNz * v, v = l + L * c; natural element e.
+struct mr_ld_pad_sx {
+    static constexpr bool kElemFast = true;
+    const float2 *x, *S;
+    long long N, NyNz;
+    int Nx, Mx, cnx, L, coil0;
+    __device__ int nsub() const { return 1; }
+    __device__ float2 operator()(long long p, int e, int) const
+    {
+        int i = e + cnx;
+        if (i >= Mx) i -= Mx;
+        if (i >= Nx) return make_float2(0.f, 0.f);
+        const long long v = p / NyNz, yz = p - v * NyNz;
+        const int l = (int)(v % L), c = (int)(v / L);
+        const long long ii = i + (long long)Nx * yz;
+        return cmul(S[ii + N * (coil0 + c)], x[ii + N * l]);
+    }
+};
+
+// I1 load: pencil p = yz + NyNz * l, round `sub` = coil of the group.
+struct mr_ld_row_coil {
+    static constexpr bool kElemFast = true;
+    const float2 *in;
+    long long NyNz;
+    int Mx, L, ncoil;
+    __device__ int nsub() const { return ncoil; }
+    __device__ float2 operator()(long long p, int e, int sub) const
+    {
+        return in[e + (long long)Mx * (p + NyNz * (long long)L * sub)];
+    }
+};
+
+// I1 store: crop, conj(S_c) and coil sum in LDS; flush writes AhAx.
+struct mr_st_crop_conj_sum {
+    static constexpr bool kElemFast = true;
+    float2 *out;
+    const float2 *S;
+    long long N, NyNz;
+    int Nx, Mx, cnx, coil0, accumulate;
+    __device__ void put(long long p, int pl, int e, int sub, float2 v, float2 *acc) const
+    {
+        int i = e + cnx;
+        if (i >= Mx) i -= Mx;
+        if (i >= Nx) return;
+        const long long yz = p % NyNz;
+        const float2 s = S[i + (long long)Nx * yz + N * (coil0 + sub)];
+        const float2 t = cmulc(s, v);
+        float2 *a = acc + (size_t)pl * Nx + i;
+        *a = (sub == 0) ? t : cadd(*a, t);
+    }
+    __device__ void flush(long long p0, int tp, float2 *acc) const
+    {
+        for (int q = threadIdx.x; q < tp * Nx; q += blockDim.x) {
+            const int pl = q / Nx, i = q - pl * Nx;
+            const long long p = p0 + pl;
+            const long long l = p / NyNz, yz = p - l * NyNz;
+            const long long o = i + (long long)Nx * yz + N * l;
+            const float2 a = acc[q];
+            out[o] = accumulate ? cadd(out[o], a) : a;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Block-diagonal multiply
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ long long mr_nat_to_cent_lin(long long loc, const mr_dims &d)
+{
+    const int j0 = (int)(loc % d.m[0]);
+    const long long q = loc / d.m[0];
+    const int j1 = (int)(q % d.m[1]);
+    const int j2 = (int)(q / d.m[1]);
+    int k0 = j0 + d.c_m[0]; if (k0 >= d.m[0]) k0 -= d.m[0];
+    int k1 = j1 + d.c_m[1]; if (k1 >= d.m[1]) k1 -= d.m[1];
+    int k2 = j2 + d.c_m[2]; if (k2 >= d.m[2]) k2 -= d.m[2];
+    return k0 + (long long)d.m[0] * (k1 + (long long)d.m[1] * k2);
+}
+
+// y_c = sum_r v_r A_rc over NC vectors, walking the stored triangle in packed
+// order once: packed element (r, c) adds A_rc v_r to y_c and, off the
+// diagonal, conj(A_rc) v_c to y_r.
+template <int L, int NC>
+__device__ __forceinline__ void mr_bd_walk(const float2 (&v)[NC][L], float2 (&y)[NC][L],
+                                           const float2 *bp, long long stride, int upper)
+{
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+#pragma unroll
+        for (int i = 0; i < L; ++i) y[k][i] = make_float2(0.f, 0.f);
+#define MR_BD_EL(r, c)                                                         \
+    do {                                                                       \
+        const float2 a = *bp;                                                  \
+        bp += stride;                                                          \
+        _Pragma("unroll") for (int k = 0; k < NC; ++k) {                       \
+            y[k][c] = cadd(y[k][c], cmul(a, v[k][r]));                         \
+            if ((r) != (c)) y[k][r] = cadd(y[k][r], cmulc(a, v[k][c]));        \
+        }                                                                      \
+    } while (0)
+    if (upper) {
+#pragma unroll
+        for (int c = 0; c < L; ++c) {
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int r = 0; r <= c; ++r) MR_BD_EL(r, c);
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < L; ++c) {
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int r = c; r < L; ++r) MR_BD_EL(r, c);
+        }
+    }
+#undef MR_BD_EL
+}
+
+// In place over K [P, L, ncoil] (natural order). Unmasked: kernel [P, T]
+// centered. Masked: inv[loc] is the kernel row of natural location loc or -1.
+template <int L, bool MASKED>
+__global__ void __launch_bounds__(256)
+mr_k_bd(float2 *__restrict__ K, const float2 *__restrict__ ker, const int *__restrict__ inv,
+        long long kstride, mr_dims d, int ncoil)
+{
+    const long long loc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (loc >= d.pad_size) return;
+    const long long P = d.pad_size;
+    const long long slot = P * L;
+    const float2 *base;
+    if (MASKED) {
+        const int m = inv[loc];
+        if (m < 0) {
+            for (int c = 0; c < ncoil; ++c)
+#pragma unroll
+                for (int l = 0; l < L; ++l) K[loc + P * l + slot * c] = make_float2(0.f, 0.f);
+            return;
+        }
+        base = ker + m;
+    } else {
+        base = ker + mr_nat_to_cent_lin(loc, d);
+    }
+    int c = 0;
+    constexpr bool PAIR = (L <= 12);
+    if (PAIR) {
+        for (; c + 1 < ncoil; c += 2) {
+            float2 *g = K + loc + slot * c;
+            float2 v[2][L], y[2][L];
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                v[0][l] = g[P * l];
+                v[1][l] = g[slot + P * l];
+            }
+            mr_bd_walk<L, 2>(v, y, base, kstride, d.tri_upper);
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                g[P * l] = y[0][l];
+                g[slot + P * l] = y[1][l];
+            }
+        }
+    }
+    for (; c < ncoil; ++c) {
+        float2 *g = K + loc + slot * c;
+        float2 v[1][L], y[1][L];
+#pragma unroll
+        for (int l = 0; l < L; ++l) v[0][l] = g[P * l];
+        mr_bd_walk<L, 1>(v, y, base, kstride, d.tri_upper);
+#pragma unroll
+        for (int l = 0; l < L; ++l) g[P * l] = y[0][l];
+    }
+}
+
+// Any rank: the rank vector of each location is staged in LDS (blockDim * L
+// values), outputs are formed one column at a time from the packed triangle.
+template <bool MASKED>
+__global__ void __launch_bounds__(64)
+mr_k_bd_any(float2 *__restrict__ K, const float2 *__restrict__ ker, const int *__restrict__ inv,
+            long long kstride, mr_dims d, int ncoil)
+{
+    extern __shared__ __align__(16) unsigned char mr_smem[];
+    const int L = d.L;
+    float2 *v = reinterpret_cast<float2 *>(mr_smem) + (size_t)threadIdx.x * L;
+    const long long loc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (loc >= d.pad_size) return;
+    const long long P = d.pad_size;
+    const float2 *base;
+    if (MASKED) {
+        const int m = inv[loc];
+        if (m < 0) {
+            for (int c = 0; c < ncoil; ++c)
+                for (int l = 0; l < L; ++l) K[loc + P * (l + (long long)L * c)] = make_float2(0.f, 0.f);
+            return;
+        }
+        base = ker + m;
+    } else {
+        base = ker + mr_nat_to_cent_lin(loc, d);
+    }
+    for (int c = 0; c < ncoil; ++c) {
+        float2 *g = K + loc + P * (long long)L * c;
+        for (int l = 0; l < L; ++l) v[l] = g[P * l];
+        for (int i = 0; i < L; ++i) {
+            float2 acc = make_float2(0.f, 0.f);
+            for (int j = 0; j < L; ++j) {
+                // element (j, i) of the block
+                int r = j, cc = i, cj = 0;
+                if (d.tri_upper) { if (r > cc) { int t = r; r = cc; cc = t; cj = 1; } }
+                else { if (r < cc) { int t = r; r = cc; cc = t; cj = 1; } }
+                const int t = d.tri_upper ? cc * (cc + 1) / 2 + r : cc * L - cc * (cc - 1) / 2 + (r - cc);
+                const float2 a = base[kstride * t];
+                acc = cadd(acc, cj ? cmulc(a, v[j]) : cmul(a, v[j]));
+            }
+            g[P * i] = acc;
+        }
+    }
+}
+
+// inv[natural location] = mask row, -1 elsewhere (mask holds centered
+// linear indices; src/cuda/mr_kernels.cu:259-281).
+__global__ void mr_k_fill_int(int *p, long long n, int v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void mr_k_mask_inverse(int *inv, const int64_t *mask, long long nm, mr_dims d)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nm) return;
+    long long k = mask[t];
+    if (k < 0 || k >= d.pad_size) return;
+    const int k0 = (int)(k % d.m[0]);
+    const long long q = k / d.m[0];
+    const int k1 = (int)(q % d.m[1]);
+    const int k2 = (int)(q / d.m[1]);
+    int j0 = k0 - d.c_m[0]; if (j0 < 0) j0 += d.m[0];
+    int j1 = k1 - d.c_m[1]; if (j1 < 0) j1 += d.m[1];
+    int j2 = k2 - d.c_m[2]; if (j2 < 0) j2 += d.m[2];
+    inv[j0 + (long long)d.m[0] * (j1 + (long long)d.m[1] * j2)] = (int)t;
+}
+
+typedef void (*mr_bd_fn)(float2 *, const float2 *, const int *, long long, mr_dims, int);
+
+template <bool MASKED> static mr_bd_fn mr_bd_pick(int L)
+{
+    switch (L) {
+#define MR_BD_CASE(n) case n: return mr_k_bd<n, MASKED>;
+        MR_BD_CASE(1) MR_BD_CASE(2) MR_BD_CASE(3) MR_BD_CASE(4)
+        MR_BD_CASE(5) MR_BD_CASE(6) MR_BD_CASE(7) MR_BD_CASE(8)
+        MR_BD_CASE(9) MR_BD_CASE(10) MR_BD_CASE(11) MR_BD_CASE(12)
+        MR_BD_CASE(13) MR_BD_CASE(14) MR_BD_CASE(15) MR_BD_CASE(16)
+#undef MR_BD_CASE
+    default: return nullptr;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plan
+// ---------------------------------------------------------------------------
+
+// Optional per-stage timing (MR_AHA_HIP_PROFILE=1, read at every execute):
+// an event is recorded on the plan's stream after every stage launch, so the
+// interval between consecutive events is that stage's kernel time.
+enum { MR_ST_F1, MR_ST_F2, MR_ST_F3, MR_ST_BD, MR_ST_I3, MR_ST_I2, MR_ST_I1, MR_ST_OTHER,
+       MR_N_ST };
+
+struct mr_prof {
+    std::vector<hipEvent_t> ev;   // ev[0] start of an execute, then one per stage
+    std::vector<int> stage;       // stage of ev[i+1] relative to ev[i]; -1 = new execute
+    double ms[MR_N_ST] = {0};
+    int calls[MR_N_ST] = {0};
+};
+
+struct mr_aha_plan_s {
+    mr_config cfg;
+    mr_dims dims;
+    hipStream_t stream;
+    int group;
+    float2 *bufA;
+    float2 *bufB;
+    int *d_inv;
+    mr_fft_len<float> fx, fy, fz;
+    mr_prof prof;
+    bool prof_on;
+};
+
+static int mr_prof_mark(mr_aha_plan p, int stage)
+{
+    if (!p->prof_on) return MR_SUCCESS;
+    hipEvent_t e;
+    MR_HIP_CHECK(hipEventCreate(&e));
+    p->prof.ev.push_back(e);
+    p->prof.stage.push_back(stage);
+    MR_HIP_CHECK(hipEventRecord(e, p->stream));
+    return MR_SUCCESS;
+}
+
+// Fold the recorded intervals into the per-stage totals (synchronises).
+static int mr_prof_collect(mr_aha_plan p)
+{
+    mr_prof &f = p->prof;
+    if (f.ev.empty()) return MR_SUCCESS;
+    MR_HIP_CHECK(hipEventSynchronize(f.ev.back()));
+    for (size_t i = 1; i < f.ev.size(); ++i) {
+        const int st = f.stage[i];
+        if (st < 0) continue;
+        float t = 0.f;
+        MR_HIP_CHECK(hipEventElapsedTime(&t, f.ev[i - 1], f.ev[i]));
+        f.ms[st] += t;
+        f.calls[st] += 1;
+    }
+    for (hipEvent_t e : f.ev) (void)hipEventDestroy(e);
+    f.ev.clear();
+    f.stage.clear();
+    return MR_SUCCESS;
+}
+
+extern "C" MR_EXPORT int mr_aha_plan_profile(mr_aha_plan p, double *ms, int *calls, int n)
+{
+    mr_clear_error();
+    MR_REQUIRE(p != nullptr, "mr_aha_plan_profile: plan is NULL");
+    MR_TRY(mr_prof_collect(p));
+    for (int i = 0; i < n && i < MR_N_ST; ++i) {
+        if (ms) ms[i] = p->prof.ms[i];
+        if (calls) calls[i] = p->prof.calls[i];
+        p->prof.ms[i] = 0.0;
+        p->prof.calls[i] = 0;
+    }
+    return MR_SUCCESS;
+}
+
+int mr_push_device(const mr_config *cfg, int *prev)
+{
+    *prev = -1;
+    if (cfg->gpu_device_id < 0) return MR_SUCCESS;
+    int cur = 0;
+    MR_HIP_CHECK(hipGetDevice(&cur));
+    if (cur != cfg->gpu_device_id) {
+        int count = 0;
+        MR_HIP_CHECK(hipGetDeviceCount(&count));
+        MR_REQUIRE(cfg->gpu_device_id < count, "device %d is not visible (%d visible)",
+                   cfg->gpu_device_id, count);
+        MR_HIP_CHECK(hipSetDevice(cfg->gpu_device_id));
+        *prev = cur;
+    }
+    return MR_SUCCESS;
+}
+
+void mr_pop_device(int prev)
+{
+    if (prev >= 0) (void)hipSetDevice(prev);
+}
+
+int mr_mem_budget(const mr_config *cfg, size_t *budget)
+{
+    size_t fr = 0, tot = 0;
+    MR_HIP_CHECK(hipMemGetInfo(&fr, &tot));
+    double f = cfg->mem_fraction;
+    if (!(f > 0.0)) f = 0.5;
+    if (f > 0.95) f = 0.95;
+    *budget = (size_t)(f * (double)fr);
+    return MR_SUCCESS;
+}
+
+static void mr_plan_free_buffers(mr_aha_plan p)
+{
+    for (hipEvent_t e : p->prof.ev) (void)hipEventDestroy(e);
+    p->prof.ev.clear();
+    p->prof.stage.clear();
+    if (p->bufA) (void)hipFree(p->bufA);
+    if (p->bufB) (void)hipFree(p->bufB);
+    if (p->d_inv) (void)hipFree(p->d_inv);
+    p->bufA = p->bufB = nullptr;
+    p->d_inv = nullptr;
+}
+
+static int mr_plan_create_impl(const mr_config *cfg, mr_aha_plan *out)
+{
+    MR_TRY(mr_config_validate(cfg));
+    mr_aha_plan p = new (std::nothrow) mr_aha_plan_s();
+    if (!p) MR_FAIL(MR_ERR_ALLOC, "failed to allocate the plan struct");
+    p->cfg = *cfg;
+    p->stream = (hipStream_t)cfg->stream;
+    mr_dims_from_config(cfg, &p->dims);
+    int rc = mr_fft_get<float>(cfg->phi2_dim[0], &p->fx);
+    if (rc == MR_SUCCESS) rc = mr_fft_get<float>(cfg->phi2_dim[1], &p->fy);
+    if (rc == MR_SUCCESS) rc = mr_fft_get<float>(cfg->phi2_dim[2], &p->fz);
+    if (rc != MR_SUCCESS) { delete p; return rc; }
+
+    // Coil group: as many coils as fit in mem_fraction of the free memory
+    // (MR_AHA_HIP_COIL_GROUP caps it). Two ping-pong grids of P*L*G values.
+    size_t budget = 0;
+    rc = mr_mem_budget(cfg, &budget);
+    if (rc != MR_SUCCESS) { delete p; return rc; }
+    const long long want = mr_env_ll("MR_AHA_HIP_COIL_GROUP", cfg->n_coils);
+    int group = (int)((want < 1) ? 1 : (want > cfg->n_coils ? cfg->n_coils : want));
+    const size_t per_coil = 2 * (size_t)p->dims.pad_size * cfg->L * sizeof(float2);
+    const size_t inv_bytes = cfg->is_masked ? (size_t)p->dims.pad_size * sizeof(int) : 0;
+    while (group > 1 && per_coil * group + inv_bytes > budget) --group;
+    for (;; --group) {
+        const size_t bytes = (size_t)p->dims.pad_size * cfg->L * group * sizeof(float2);
+        hipError_t e = hipMalloc((void **)&p->bufA, bytes);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->bufB, bytes);
+        if (e == hipSuccess && inv_bytes) e = hipMalloc((void **)&p->d_inv, inv_bytes);
+        if (e == hipSuccess) break;
+        (void)hipGetLastError();
+        mr_plan_free_buffers(p);
+        if (group == 1) {
+            delete p;
+            MR_FAIL(MR_ERR_ALLOC, "failed to allocate %zu bytes of work buffers for one coil: %s",
+                    2 * bytes + inv_bytes, hipGetErrorString(e));
+        }
+    }
+    p->group = group;
+    mr_clear_error();
+    if (cfg->verbose)
+        printf("[mr_aha] plan: %d coil(s) per group, grid [%d %d %d], L = %d\n", group,
+               cfg->phi2_dim[0], cfg->phi2_dim[1], cfg->phi2_dim[2], cfg->L);
+    *out = p;
+    return MR_SUCCESS;
+}
+
+int mr_aha_plan_group(mr_aha_plan p) { return p->group; }
+
+extern "C" MR_EXPORT int mr_aha_plan_create(const mr_config *cfg, mr_aha_plan *plan_out)
+{
+    mr_clear_error();
+    MR_REQUIRE(plan_out != nullptr, "mr_aha_plan_create: plan_out is NULL");
+    *plan_out = nullptr;
+    MR_REQUIRE(cfg != nullptr, "mr_aha_plan_create: cfg is NULL");
+    int prev = -1;
+    int rc = mr_push_device(cfg, &prev);
+    if (rc == MR_SUCCESS) rc = mr_plan_create_impl(cfg, plan_out);
+    mr_pop_device(prev);
+    return rc;
+}
+
+extern "C" MR_EXPORT int mr_aha_plan_destroy(mr_aha_plan plan)
+{
+    mr_clear_error();
+    if (!plan) return MR_SUCCESS;
+    int prev = -1;
+    (void)mr_push_device(&plan->cfg, &prev);
+    mr_plan_free_buffers(plan);
+    mr_pop_device(prev);
+    delete plan;
+    return MR_SUCCESS;
+}
+
+static int mr_blockdiag(mr_aha_plan p, float2 *K, int nc, const float2 *ker)
+{
+    const mr_dims d = p->dims;
+    const bool masked = p->cfg.is_masked != 0;
+    const long long kstride = masked ? (long long)p->cfg.num_masked : d.pad_size;
+    mr_bd_fn fn = masked ? mr_bd_pick<true>(d.L) : mr_bd_pick<false>(d.L);
+    if (fn) {
+        const long long blocks = mr_ceil_div(d.pad_size, 256);
+        hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(256), 0, p->stream, K, ker, p->d_inv,
+                           kstride, d, nc);
+        MR_LAUNCH_CHECK("mr_k_bd");
+        return MR_SUCCESS;
+    }
+    const size_t lds = (size_t)64 * d.L * sizeof(float2);
+    if (lds > 64 * 1024)
+        MR_FAIL(MR_ERR_UNSUPPORTED, "L = %d needs %zu bytes of LDS per 64 locations", d.L, lds);
+    const long long blocks = mr_ceil_div(d.pad_size, 64);
+    if (masked)
+        hipLaunchKernelGGL(mr_k_bd_any<true>, dim3((unsigned)blocks), dim3(64), lds, p->stream, K,
+                           ker, p->d_inv, kstride, d, nc);
+    else
+        hipLaunchKernelGGL(mr_k_bd_any<false>, dim3((unsigned)blocks), dim3(64), lds, p->stream, K,
+                           ker, p->d_inv, kstride, d, nc);
+    MR_LAUNCH_CHECK("mr_k_bd_any");
+    return MR_SUCCESS;
+}
+
+static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, const float2 *ker,
+                           const int64_t *mask, float2 *out)
+{
+    const mr_config &cfg = p->cfg;
+    const mr_dims d = p->dims;
+    hipStream_t s = p->stream;
+    const int L = cfg.L;
+    const int Nx = d.n[0], Ny = d.n[1], Nz = d.n[2];
+    const int Mx = d.m[0], My = d.m[1], Mz = d.m[2];
+    const long long N = d.image_size;
+    const long long NyNz = (long long)Ny * Nz;
+    const bool three = cfg.n_spatial == 3;
+    p->prof_on = mr_env_ll("MR_AHA_HIP_PROFILE", 0) != 0;
+    MR_TRY(mr_prof_mark(p, -1));
+
+    if (cfg.is_masked) {
+        const long long P = d.pad_size, nm = cfg.num_masked;
+        hipLaunchKernelGGL(mr_k_fill_int, dim3((unsigned)mr_ceil_div(P, 256)), dim3(256), 0, s,
+                           p->d_inv, P, -1);
+        MR_LAUNCH_CHECK("mr_k_fill_int");
+        hipLaunchKernelGGL(mr_k_mask_inverse, dim3((unsigned)mr_ceil_div(nm, 256)), dim3(256), 0,
+                           s, p->d_inv, mask, nm, d);
+        MR_LAUNCH_CHECK("mr_k_mask_inverse");
+        MR_TRY(mr_prof_mark(p, MR_ST_OTHER));
+    }
+
+    for (int coil0 = 0; coil0 < cfg.n_coils; coil0 += p->group) {
+        const int nc = (cfg.n_coils - coil0 < p->group) ? cfg.n_coils - coil0 : p->group;
+        const long long V = (long long)L * nc;
+        float2 *A = p->bufA, *B = p->bufB;
+
+        // F1: pad * S, x-FFT -> A [Mx, Ny, Nz, V]
+        {
+            mr_ld_pad_sx ld{x, S, N, NyNz, Nx, Mx, d.c_n[0], L, coil0};
+            mr_st_axis<float2, true> st{A, 1, Mx};
+            MR_TRY((mr_fft_pass<float, false>(p->fx, ld, st, NyNz * V, 0, 4, s, "F1")));
+            MR_TRY(mr_prof_mark(p, MR_ST_F1));
+        }
+        // F2: y-FFT -> B [Mx, My, Nz, V]
+        {
+            mr_ld_pad<float2, false> ld{A, Mx, My, Ny, d.c_n[1]};
+            mr_st_axis<float2, false> st{B, Mx, My};
+            MR_TRY((mr_fft_pass<float, false>(p->fy, ld, st, (long long)Mx * Nz * V, 0, 16, s, "F2")));
+            MR_TRY(mr_prof_mark(p, MR_ST_F2));
+        }
+        float2 *K = B, *free_buf = A;
+        if (three) {
+            // F3: z-FFT -> A [Mx, My, Mz, V]
+            mr_ld_pad<float2, false> ld{B, (long long)Mx * My, Mz, Nz, d.c_n[2]};
+            mr_st_axis<float2, false> st{A, (long long)Mx * My, Mz};
+            MR_TRY((mr_fft_pass<float, false>(p->fz, ld, st, (long long)Mx * My * V, 0, 16, s, "F3")));
+            MR_TRY(mr_prof_mark(p, MR_ST_F3));
+            K = A;
+            free_buf = B;
+        }
+        MR_TRY(mr_blockdiag(p, K, nc, ker));
+        MR_TRY(mr_prof_mark(p, MR_ST_BD));
+        float2 *src = K;
+        if (three) {
+            // I3: inverse z-FFT, keep Nz planes -> free_buf [Mx, My, Nz, V]
+            mr_ld_axis<float2, false> ld{K, (long long)Mx * My, Mz};
+            mr_st_crop<float2, false> st{free_buf, (long long)Mx * My, Mz, Nz, d.c_n[2]};
+            MR_TRY((mr_fft_pass<float, true>(p->fz, ld, st, (long long)Mx * My * V, 0, 16, s, "I3")));
+            MR_TRY(mr_prof_mark(p, MR_ST_I3));
+            src = free_buf;
+            free_buf = K;
+        }
+        // I2: inverse y-FFT, keep Ny rows -> [Mx, Ny, Nz, V]
+        float2 *dst = (src == A) ? B : A;
+        {
+            mr_ld_axis<float2, false> ld{src, Mx, My};
+            mr_st_crop<float2, false> st{dst, Mx, My, Ny, d.c_n[1]};
+            MR_TRY((mr_fft_pass<float, true>(p->fy, ld, st, (long long)Mx * Nz * V, 0, 16, s, "I2")));
+            MR_TRY(mr_prof_mark(p, MR_ST_I2));
+        }
+        // I1: inverse x-FFT, crop, conj(S) and coil sum -> out
+        {
+            mr_ld_row_coil ld{dst, NyNz, Mx, L, nc};
+            mr_st_crop_conj_sum st{out, S, N, NyNz, Nx, Mx, d.c_n[0], coil0, coil0 > 0 ? 1 : 0};
+            MR_TRY((mr_fft_pass<float, true>(p->fx, ld, st, NyNz * L, (size_t)Nx, 4, s, "I1")));
+            MR_TRY(mr_prof_mark(p, MR_ST_I1));
+        }
+    }
+    if (cfg.verbose) {
+        MR_HIP_CHECK(hipStreamSynchronize(s));
+        printf("[mr_aha] AhA applied over %d coil(s) in groups of %d, grid [%d %d %d] -> "
+               "[%d %d %d], L = %d%s\n",
+               cfg.n_coils, p->group, Nx, Ny, Nz, Mx, My, Mz, L, cfg.is_masked ? ", masked" : "");
+    }
+    return MR_SUCCESS;
+}
+
+int mr_aha_plan_execute_raw(mr_aha_plan plan, const mr_complex *d_x, const mr_complex *d_SEs,
+                            const mr_complex *d_phi2_toep, const int64_t *d_mask,
+                            mr_complex *d_AhAx)
+{
+    return mr_execute_impl(plan, reinterpret_cast<const float2 *>(d_x),
+                           reinterpret_cast<const float2 *>(d_SEs),
+                           reinterpret_cast<const float2 *>(d_phi2_toep), d_mask,
+                           reinterpret_cast<float2 *>(d_AhAx));
+}
+
+extern "C" MR_EXPORT int mr_aha_plan_execute(mr_aha_plan plan, const mr_complex *d_x,
+                                             const mr_complex *d_SEs,
+                                             const mr_complex *d_phi2_toep, const int64_t *d_mask,
+                                             mr_complex *d_AhAx)
+{
+    mr_clear_error();
+    MR_REQUIRE(plan != nullptr, "mr_aha_plan_execute: plan is NULL");
+    MR_REQUIRE(d_x != nullptr, "mr_aha_plan_execute: d_x is NULL");
+    MR_REQUIRE(d_SEs != nullptr, "mr_aha_plan_execute: d_SEs is NULL");
+    MR_REQUIRE(d_phi2_toep != nullptr, "mr_aha_plan_execute: d_phi2_toep is NULL");
+    MR_REQUIRE(d_AhAx != nullptr, "mr_aha_plan_execute: d_AhAx is NULL");
+    MR_REQUIRE((const void *)d_AhAx != (const void *)d_x,
+               "mr_aha_plan_execute: d_AhAx must not alias d_x");
+    if (plan->cfg.is_masked)
+        MR_REQUIRE(d_mask != nullptr,
+                   "mr_aha_plan_execute: d_mask is required when is_masked is set");
+    int prev = -1;
+    int rc = mr_push_device(&plan->cfg, &prev);
+    if (rc == MR_SUCCESS)
+        rc = mr_aha_plan_execute_raw(plan, d_x, d_SEs, d_phi2_toep, d_mask, d_AhAx);
+    mr_pop_device(prev);
+    return rc;
+}
+
+extern "C" MR_EXPORT int mr_aha_toeplitz(const mr_config *cfg, const mr_complex *d_x,
+                                         const mr_complex *d_SEs, const mr_complex *d_phi2_toep,
+                                         const int64_t *d_mask, mr_complex *d_AhAx)
+{
+    mr_aha_plan plan = nullptr;
+    int rc = mr_aha_plan_create(cfg, &plan);
+    if (rc != MR_SUCCESS) return rc;
+    rc = mr_aha_plan_execute(plan, d_x, d_SEs, d_phi2_toep, d_mask, d_AhAx);
+    char msg[512];
+    snprintf(msg, sizeof(msg), "%s", mr_last_error());
+    const int rd = mr_aha_plan_destroy(plan);
+    if (rc != MR_SUCCESS) {
+        mr_set_error("%s", msg);
+        return rc;
+    }
+    return rd;
+}

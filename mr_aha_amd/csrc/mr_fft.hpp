@@ -1,0 +1,387 @@
+// Project FFT engine for gfx950: batched 1-D complex transforms along one
+// axis of a strided array, one workgroup per tile of pencils staged in LDS.
+//
+// Algorithm: self-sorting (Stockham) mixed-radix DIT. Stage s with radix R
+// and Ns = product of earlier radices maps, for every butterfly j < n/R,
+//   v_r = A[j + r n/R] * w^(k r n/(Ns R)),  k = j mod Ns,  w = e^{-+2 pi i/n}
+//   V   = DFT_R(v)
+//   B[(j div Ns) Ns R + k + q Ns] = V_q
+// (Govindaraju et al., "High performance discrete Fourier transforms on
+// graphics processors", SC 2008). Radices 2, 3, 4, 5, 7, 8 have unrolled
+// butterflies; any other prime factor uses a per-output dot product over
+// the n-th-root table, so every length works (prime grids of the test
+// fixtures) while smooth lengths take the fast butterflies.
+//
+// The transform is unnormalised; FWD uses e^{-2 pi i jk/n}, INV e^{+...}.
+// Data never leaves the workgroup between stages: a pencil is loaded once
+// through a Load functor and written once through a Store functor, which
+// is where padding, sensitivity products, crops and coil sums are fused.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#define MR_FFT_MAXST 24
+
+template <typename R> struct cvec;
+template <> struct cvec<float>  { using T = float2; };
+template <> struct cvec<double> { using T = double2; };
+
+template <typename V> __device__ __forceinline__ V cmake(decltype(V::x) a, decltype(V::x) b)
+{
+    V r; r.x = a; r.y = b; return r;
+}
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return cmake<V>(a.x + b.x, a.y + b.y); }
+template <typename V> __device__ __forceinline__ V csub(V a, V b) { return cmake<V>(a.x - b.x, a.y - b.y); }
+template <typename V> __device__ __forceinline__ V cmul(V a, V b)
+{
+    return cmake<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+template <typename V> __device__ __forceinline__ V cmulc(V a, V b)
+{
+    return cmake<V>(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+template <typename V> __device__ __forceinline__ V cconj(V a) { return cmake<V>(a.x, -a.y); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, typename V> __device__ __forceinline__ V cmul_mi(V a)
+{
+    return INV ? cmake<V>(-a.y, a.x) : cmake<V>(a.y, -a.x);
+}
+
+// Per-length plan passed by value to kernels.
+template <typename R> struct mr_fft_len {
+    int n;
+    int nst;
+    int radix[MR_FFT_MAXST];
+    const typename cvec<R>::T *tw;  // tw[q] = exp(-2 pi i q / n), q < n (device)
+};
+
+// Factor n into the stage radices: 8s first, then 4, 2, 3, 5, 7, then the
+// remaining primes in ascending order.
+inline int mr_fft_factor(int n, int *radix)
+{
+    int ns = 0;
+    int m = n;
+    const int small[] = {8, 4, 2, 3, 5, 7};
+    for (int r : small)
+        while (m % r == 0 && ns < MR_FFT_MAXST) { radix[ns++] = r; m /= r; }
+    for (int p = 11; m > 1 && ns < MR_FFT_MAXST; p += 2)
+        while (m % p == 0 && ns < MR_FFT_MAXST) { radix[ns++] = p; m /= p; }
+    return (m == 1) ? ns : -1;
+}
+
+// n is "smooth" when every radix is one of the unrolled ones.
+inline bool mr_fft_smooth(int n)
+{
+    int r[MR_FFT_MAXST];
+    int ns = mr_fft_factor(n, r);
+    if (ns < 0) return false;
+    for (int i = 0; i < ns; ++i)
+        if (r[i] > 8 || r[i] == 6) return false;
+    return true;
+}
+
+// Smallest 2^a 3^b 5^c 7^d >= n.
+inline int mr_fft_next_smooth(int n)
+{
+    for (int m = (n < 1 ? 1 : n);; ++m) {
+        int k = m;
+        for (int p : {2, 3, 5, 7}) while (k % p == 0) k /= p;
+        if (k == 1) return m;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Butterflies
+// ---------------------------------------------------------------------------
+
+template <bool INV, typename V> __device__ __forceinline__ void bfly2(V *v)
+{
+    V a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <bool INV, typename V> __device__ __forceinline__ void bfly4(V *v)
+{
+    V a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+    V c = cadd(v[1], v[3]), d = cmul_mi<INV>(csub(v[1], v[3]));
+    v[0] = cadd(a, c);
+    v[2] = csub(a, c);
+    v[1] = cadd(b, d);
+    v[3] = csub(b, d);
+}
+
+template <bool INV, typename V> __device__ __forceinline__ void bfly8(V *v)
+{
+    using R = decltype(V::x);
+    V e[4] = {v[0], v[2], v[4], v[6]};
+    V o[4] = {v[1], v[3], v[5], v[7]};
+    bfly4<INV>(e);
+    bfly4<INV>(o);
+    const R h = (R)0.70710678118654752440;
+    // w8^q for q = 0..3: 1, (1 -+ i)/sqrt2, -+i, (-1 -+ i)/sqrt2
+    V t1 = INV ? cmake<V>(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y))
+               : cmake<V>(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+    V t2 = cmul_mi<INV>(o[2]);
+    V t3 = INV ? cmake<V>(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y))
+               : cmake<V>(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+    v[0] = cadd(e[0], o[0]);  v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], t1);    v[5] = csub(e[1], t1);
+    v[2] = cadd(e[2], t2);    v[6] = csub(e[2], t2);
+    v[3] = cadd(e[3], t3);    v[7] = csub(e[3], t3);
+}
+
+// Odd prime radix with compile-time constants: V_q = sum_r v_r e^{-+2 pi i r q/R},
+// written with the symmetric pairs (r, R-r) to halve the multiplies.
+template <int RR> struct mr_trig;
+template <> struct mr_trig<3> {
+    static constexpr double c[3] = {1.0, -0.5, -0.5};
+    static constexpr double s[3] = {0.0, 0.86602540378443864676, -0.86602540378443864676};
+};
+template <> struct mr_trig<5> {
+    static constexpr double c[5] = {1.0, 0.30901699437494742410, -0.80901699437494742410,
+                                    -0.80901699437494742410, 0.30901699437494742410};
+    static constexpr double s[5] = {0.0, 0.95105651629515357212, 0.58778525229247312917,
+                                    -0.58778525229247312917, -0.95105651629515357212};
+};
+template <> struct mr_trig<7> {
+    static constexpr double c[7] = {1.0, 0.62348980185873353053, -0.22252093395631440429,
+                                    -0.90096886790241912624, -0.90096886790241912624,
+                                    -0.22252093395631440429, 0.62348980185873353053};
+    static constexpr double s[7] = {0.0, 0.78183148246802980871, 0.97492791218182360702,
+                                    0.43388373911755812048, -0.43388373911755812048,
+                                    -0.97492791218182360702, -0.78183148246802980871};
+};
+
+template <int RR, bool INV, typename V> __device__ __forceinline__ void bfly_odd(V *v)
+{
+    using R = decltype(V::x);
+    constexpr int H = (RR - 1) / 2;
+    V sp[H], dm[H];
+#pragma unroll
+    for (int r = 1; r <= H; ++r) {
+        sp[r - 1] = cadd(v[r], v[RR - r]);
+        dm[r - 1] = csub(v[r], v[RR - r]);
+    }
+    V out[RR];
+    V s0 = v[0];
+#pragma unroll
+    for (int r = 0; r < H; ++r) s0 = cadd(s0, sp[r]);
+    out[0] = s0;
+#pragma unroll
+    for (int q = 1; q <= H; ++q) {
+        // re part from symmetric sums with cos, im part from differences with sin
+        R ar = v[0].x, ai = v[0].y, br = 0, bi = 0;
+#pragma unroll
+        for (int r = 1; r <= H; ++r) {
+            const int k = (r * q) % RR;
+            const R c = (R)mr_trig<RR>::c[k];
+            const R s = (R)mr_trig<RR>::s[k];
+            ar += c * sp[r - 1].x;
+            ai += c * sp[r - 1].y;
+            br += s * dm[r - 1].x;
+            bi += s * dm[r - 1].y;
+        }
+        // forward: e^{-i theta}: sum = A - i B  with B = sum s*dm
+        if (!INV) {
+            out[q] = cmake<V>(ar + bi, ai - br);
+            out[RR - q] = cmake<V>(ar - bi, ai + br);
+        } else {
+            out[q] = cmake<V>(ar - bi, ai + br);
+            out[RR - q] = cmake<V>(ar + bi, ai - br);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RR; ++q) v[q] = out[q];
+}
+
+template <int RR, bool INV, typename V> __device__ __forceinline__ void bfly(V *v)
+{
+    if constexpr (RR == 2) bfly2<INV>(v);
+    else if constexpr (RR == 4) bfly4<INV>(v);
+    else if constexpr (RR == 8) bfly8<INV>(v);
+    else bfly_odd<RR, INV>(v);
+}
+
+// One Stockham stage with compile-time radix, LDS A -> LDS B, over tp pencils
+// (pencil stride ns) of length n, Ns = product of earlier radices.
+template <int RR, bool INV, typename V>
+__device__ __forceinline__ void mr_stage(const V *__restrict__ A, V *__restrict__ B, const V *tw,
+                                         int n, int ns, int tp, int Ns)
+{
+    const int m = n / RR;
+    const int stride = n / (Ns * RR);
+    for (int f = threadIdx.x; f < tp * m; f += blockDim.x) {
+        const int p = f / m;
+        const int j = f - p * m;
+        const int k = j % Ns;
+        const V *a = A + p * ns + j;
+        V v[RR];
+#pragma unroll
+        for (int r = 0; r < RR; ++r) v[r] = a[r * m];
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < RR; ++r) {
+                V w = tw[k * r * stride];
+                v[r] = INV ? cmulc(w, v[r]) : cmul(w, v[r]);
+            }
+        }
+        bfly<RR, INV>(v);
+        V *b = B + p * ns + (j / Ns) * Ns * RR + k;
+#pragma unroll
+        for (int q = 0; q < RR; ++q) b[q * Ns] = v[q];
+    }
+}
+
+// Any radix: each thread evaluates one output of one butterfly as a length-R
+// dot product; the combined stage and DFT exponent is r (k + Ns q) stride.
+template <bool INV, typename V>
+__device__ __forceinline__ void mr_stage_any(const V *__restrict__ A, V *__restrict__ B,
+                                             const V *tw, int n, int ns, int tp, int Ns, int RR)
+{
+    const int m = n / RR;
+    const int stride = n / (Ns * RR);
+    const int per = m * RR;  // outputs per pencil
+    for (int f = threadIdx.x; f < tp * per; f += blockDim.x) {
+        const int p = f / per;
+        const int rem = f - p * per;
+        const int j = rem / RR;
+        const int q = rem - j * RR;
+        const int k = j % Ns;
+        const V *a = A + p * ns + j;
+        const long long base = (long long)(k + Ns * q) * stride;
+        V acc = cmake<V>(0, 0);
+        for (int r = 0; r < RR; ++r) {
+            V w = tw[(int)((base * r) % n)];
+            acc = cadd(acc, INV ? cmulc(w, a[r * m]) : cmul(w, a[r * m]));
+        }
+        B[p * ns + (j / Ns) * Ns * RR + k + q * Ns] = acc;
+    }
+}
+
+// Full transform of tp pencils held in LDS buffer A (B is scratch of the
+// same size). Returns the buffer that holds the result.
+template <bool INV, typename R>
+__device__ typename cvec<R>::T *mr_fft_lds(const mr_fft_len<R> &f, typename cvec<R>::T *A,
+                                           typename cvec<R>::T *B, const typename cvec<R>::T *tw,
+                                           int ns, int tp)
+{
+    using V = typename cvec<R>::T;
+    int Ns = 1;
+    for (int s = 0; s < f.nst; ++s) {
+        const int RR = f.radix[s];
+        switch (RR) {
+        case 2: mr_stage<2, INV, V>(A, B, tw, f.n, ns, tp, Ns); break;
+        case 3: mr_stage<3, INV, V>(A, B, tw, f.n, ns, tp, Ns); break;
+        case 4: mr_stage<4, INV, V>(A, B, tw, f.n, ns, tp, Ns); break;
+        case 5: mr_stage<5, INV, V>(A, B, tw, f.n, ns, tp, Ns); break;
+        case 7: mr_stage<7, INV, V>(A, B, tw, f.n, ns, tp, Ns); break;
+        case 8: mr_stage<8, INV, V>(A, B, tw, f.n, ns, tp, Ns); break;
+        default: mr_stage_any<INV, V>(A, B, tw, f.n, ns, tp, Ns, RR); break;
+        }
+        __syncthreads();
+        V *t = A; A = B; B = t;
+        Ns *= RR;
+    }
+    return A;
+}
+
+// LDS pencil stride: odd for even n so that strided LDS accesses of
+// neighbouring pencils fall in different banks.
+__host__ __device__ inline int mr_fft_ns(int n) { return (n % 2 == 0) ? n + 1 : n; }
+
+// Generic axis kernel. Load::kElemFast selects the global access order
+// (true: consecutive threads walk one pencil; false: consecutive threads
+// walk consecutive pencils at one element). Load/Store see the global pencil
+// index and the element index in natural order.
+//
+// Loads may run in several rounds (Load::nsub > 1, e.g. one per coil); each
+// round is transformed and handed to Store::put(p, e, sub, value), then
+// Store::flush(p, local_p, tp) runs once after the last round.
+template <typename R, bool INV, class Load, class Store>
+__global__ void __launch_bounds__(256)
+mr_fft_axis_kernel(mr_fft_len<R> f, Load ld, Store st, long long npencil, int TX)
+{
+    using V = typename cvec<R>::T;
+    extern __shared__ __align__(16) unsigned char mr_smem[];
+    const int n = f.n;
+    const int ns = mr_fft_ns(n);
+    V *tw = reinterpret_cast<V *>(mr_smem);
+    V *A0 = tw + n;
+    V *B0 = A0 + (size_t)TX * ns;
+    V *acc = B0 + (size_t)TX * ns;  // Store scratch (TX * Store::acc_len)
+    const long long p0 = (long long)blockIdx.x * TX;
+    const int tp = (int)((npencil - p0 < TX) ? (npencil - p0) : TX);
+
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = f.tw[i];
+
+    const int nsub = ld.nsub();
+    for (int sub = 0; sub < nsub; ++sub) {
+        V *A = A0, *B = B0;
+        if (Load::kElemFast) {
+            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
+                const int p = q / n, e = q - p * n;
+                A[p * ns + e] = ld(p0 + p, e, sub);
+            }
+        } else {
+            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
+                const int e = q / tp, p = q - e * tp;
+                A[p * ns + e] = ld(p0 + p, e, sub);
+            }
+        }
+        __syncthreads();
+        V *res = mr_fft_lds<INV, R>(f, A, B, tw, ns, tp);
+        if (Store::kElemFast) {
+            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
+                const int p = q / n, e = q - p * n;
+                st.put(p0 + p, p, e, sub, res[p * ns + e], acc);
+            }
+        } else {
+            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
+                const int e = q / tp, p = q - e * tp;
+                st.put(p0 + p, p, e, sub, res[p * ns + e], acc);
+            }
+        }
+        __syncthreads();
+    }
+    st.flush(p0, tp, acc);
+}
+
+// ---------------------------------------------------------------------------
+// Host side: twiddle tables cached per (length, precision).
+// ---------------------------------------------------------------------------
+
+// Returns the device plan of length n in precision R (tables are created on
+// first use on the current device and kept until mr_fft_release_all()).
+template <typename R> int mr_fft_get(int n, mr_fft_len<R> *out);
+void mr_fft_release_all();
+
+// Workgroup tile and dynamic LDS of an axis pass.
+struct mr_fft_launch {
+    int TX;
+    size_t lds;
+    long long blocks;
+};
+template <typename R>
+inline mr_fft_launch mr_fft_pick(int n, long long npencil, size_t acc_elems_per_pencil,
+                                 int want_tx)
+{
+    using V = typename cvec<R>::T;
+    const size_t budget = 96 * 1024;
+    const int ns = mr_fft_ns(n);
+    int TX = want_tx;
+    auto need = [&](int tx) {
+        return ((size_t)n + 2 * (size_t)tx * ns + (size_t)tx * acc_elems_per_pencil) * sizeof(V);
+    };
+    while (TX > 1 && need(TX) > budget) TX /= 2;
+    if (TX > npencil) TX = (int)(npencil < 1 ? 1 : npencil);
+    mr_fft_launch l;
+    l.TX = TX;
+    l.lds = need(TX);
+    l.blocks = (npencil + TX - 1) / TX;
+    return l;
+}

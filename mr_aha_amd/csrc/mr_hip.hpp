@@ -1,0 +1,114 @@
+// HIP-side helpers shared by the device translation units.
+#pragma once
+
+#include "mr_internal.hpp"
+#include "mr_fft.hpp"
+
+#include <hip/hip_runtime.h>
+
+#define MR_HIP_CHECK(call)                                                     \
+    do {                                                                       \
+        hipError_t mr_e_ = (call);                                             \
+        if (mr_e_ != hipSuccess)                                               \
+            MR_FAIL(MR_ERR_CUDA, "%s failed at %s:%d: %s", #call, __FILE__,    \
+                    __LINE__, hipGetErrorString(mr_e_));                       \
+    } while (0)
+
+#define MR_LAUNCH_CHECK(name)                                                  \
+    do {                                                                       \
+        hipError_t mr_e_ = hipGetLastError();                                  \
+        if (mr_e_ != hipSuccess)                                               \
+            MR_FAIL(MR_ERR_CUDA, "kernel %s failed to launch at %s:%d: %s",    \
+                    name, __FILE__, __LINE__, hipGetErrorString(mr_e_));       \
+    } while (0)
+
+// Make cfg's device current (gpu_device_id < 0 keeps the caller's); *prev
+// receives the device to restore or -1.
+int mr_push_device(const mr_config *cfg, int *prev);
+void mr_pop_device(int prev);
+
+// Budget = clamp(mem_fraction) * free bytes of the current device.
+int mr_mem_budget(const mr_config *cfg, size_t *budget);
+
+// Launch one axis pass of the FFT engine on `stream`.
+template <typename R, bool INV, class Load, class Store>
+int mr_fft_pass(const mr_fft_len<R> &f, const Load &ld, const Store &st, long long npencil,
+                size_t acc_per_pencil, int want_tx, hipStream_t stream, const char *name)
+{
+    if (npencil <= 0) return MR_SUCCESS;
+    mr_fft_launch l = mr_fft_pick<R>(f.n, npencil, acc_per_pencil, want_tx);
+    if (l.blocks > 0x7fffffffLL)
+        MR_FAIL(MR_ERR_UNSUPPORTED, "%s: %lld pencils exceed the launch limit", name, npencil);
+    hipLaunchKernelGGL((mr_fft_axis_kernel<R, INV, Load, Store>), dim3((unsigned)l.blocks),
+                       dim3(256), l.lds, stream, f, ld, st, npencil, l.TX);
+    MR_LAUNCH_CHECK(name);
+    return MR_SUCCESS;
+}
+
+// Generic axis functors over an array viewed as [I, len, O] (inner stride 1).
+//   pencil p = i + I * o ; element e along the axis.
+
+// Full-length load, element e at offset i + I*(e + n*o).
+template <typename V, bool FAST> struct mr_ld_axis {
+    static constexpr bool kElemFast = FAST;
+    const V *in;
+    long long I;
+    int n;
+    __device__ int nsub() const { return 1; }
+    __device__ V operator()(long long p, int e, int) const
+    {
+        const long long o = p / I, i = p - o * I;
+        return in[i + I * (e + (long long)n * o)];
+    }
+};
+
+// Zero-padded load: natural element e of a padded axis of length M reads
+// source index s = (e + c) mod M of a compact axis of length Ns when s < Ns
+// (the centering scatter of src/cuda/mr_kernels.cu:111-128), else 0.
+template <typename V, bool FAST> struct mr_ld_pad {
+    static constexpr bool kElemFast = FAST;
+    const V *in;
+    long long I;
+    int M, Ns, c;
+    __device__ int nsub() const { return 1; }
+    __device__ V operator()(long long p, int e, int) const
+    {
+        int s = e + c;
+        if (s >= M) s -= M;
+        if (s >= Ns) return cmake<V>(0, 0);
+        const long long o = p / I, i = p - o * I;
+        return in[i + I * (s + (long long)Ns * o)];
+    }
+};
+
+// Full-length store to [I, n, O].
+template <typename V, bool FAST> struct mr_st_axis {
+    static constexpr bool kElemFast = FAST;
+    V *out;
+    long long I;
+    int n;
+    __device__ void put(long long p, int, int e, int, V v, V *) const
+    {
+        const long long o = p / I, i = p - o * I;
+        out[i + I * (e + (long long)n * o)] = v;
+    }
+    __device__ void flush(long long, int, V *) const {}
+};
+
+// Cropping store: natural element e lands at compact index s = (e + c) mod M
+// when s < Ns; other elements are discarded (crop of mr_kernels.cu:158-165).
+template <typename V, bool FAST> struct mr_st_crop {
+    static constexpr bool kElemFast = FAST;
+    V *out;
+    long long I;
+    int M, Ns, c;
+    __device__ void put(long long p, int, int e, int, V v, V *) const
+    {
+        int s = e + c;
+        if (s >= M) s -= M;
+        if (s >= Ns) return;
+        const long long o = p / I, i = p - o * I;
+        out[i + I * (s + (long long)Ns * o)] = v;
+    }
+    __device__ void flush(long long, int, V *) const {}
+};

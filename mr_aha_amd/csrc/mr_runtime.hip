@@ -1,0 +1,69 @@
+// Runtime pieces of libmr_aha_c: backend identity, HIP error hook and the
+// per-device cache of FFT twiddle tables.
+
+#include "mr_hip.hpp"
+
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+void mr_backend_clear_error() { (void)hipGetLastError(); }
+
+// "cuda" is the value that makes the reference's bindings treat every array
+// argument as a device pointer (python/mr_aha/_lib.py:700-715); this library
+// follows that device-pointer ABI on HIP devices.
+extern "C" MR_EXPORT const char *mr_backend(void) { return "cuda"; }
+
+namespace {
+std::mutex g_tw_mu;
+std::map<std::tuple<int, int, int>, void *> g_tw;  // (device, n, bytes per real) -> table
+}  // namespace
+
+template <typename R> int mr_fft_get(int n, mr_fft_len<R> *out)
+{
+    using V = typename cvec<R>::T;
+    MR_REQUIRE(n >= 1, "FFT length %d must be >= 1", n);
+    const size_t lds = ((size_t)n + 2 * (size_t)mr_fft_ns(n)) * sizeof(V);
+    if (lds > 64 * 1024)
+        MR_FAIL(MR_ERR_UNSUPPORTED, "FFT length %d exceeds the single-pencil LDS limit", n);
+    out->n = n;
+    out->nst = mr_fft_factor(n, out->radix);
+    if (out->nst < 0) MR_FAIL(MR_ERR_UNSUPPORTED, "FFT length %d has too many factors", n);
+    int dev = 0;
+    MR_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto key = std::make_tuple(dev, n, (int)sizeof(R));
+    auto it = g_tw.find(key);
+    if (it == g_tw.end()) {
+        std::vector<V> h(n);
+        for (int q = 0; q < n; ++q) {
+            // exact quadrant values, long double elsewhere
+            const long double a = -2.0L * 3.14159265358979323846264338327950288L * q / n;
+            h[q].x = (R)cosl(a);
+            h[q].y = (R)sinl(a);
+            if ((4LL * q) % n == 0) {
+                const int quad = (int)((4LL * q) / n);
+                const R cs[4] = {1, 0, -1, 0}, sn[4] = {0, -1, 0, 1};
+                h[q].x = cs[quad];
+                h[q].y = sn[quad];
+            }
+        }
+        void *d = nullptr;
+        MR_HIP_CHECK(hipMalloc(&d, n * sizeof(V)));
+        MR_HIP_CHECK(hipMemcpy(d, h.data(), n * sizeof(V), hipMemcpyHostToDevice));
+        it = g_tw.emplace(key, d).first;
+    }
+    out->tw = reinterpret_cast<const V *>(it->second);
+    return MR_SUCCESS;
+}
+
+template int mr_fft_get<float>(int, mr_fft_len<float> *);
+template int mr_fft_get<double>(int, mr_fft_len<double> *);
+
+void mr_fft_release_all()
+{
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    for (auto &kv : g_tw) (void)hipFree(kv.second);
+    g_tw.clear();
+}

@@ -1,0 +1,251 @@
+"""GPU parity of the HIP path against the FP64 oracle and the golden fixtures.
+
+Tolerances (relative 2-norm), as in the reference's tests
+(tests/check_op.py:92-94, tests/check_reference.py:42-52):
+  1e-5  HIP Toeplitz apply vs the FP64 five-stage apply on the same kernel,
+        HIP kernel (ES gridding, eps 1e-6) vs the exact Dirichlet-sum kernel,
+        HIP end-to-end vs the exact Toeplitz result
+  1e-4  HIP Toeplitz vs the exact direct-NUDFT normal operator
+  1e-6  operator object vs the stateless primitives (same method)
+  exact integer equality for masks, bit equality for an injected kernel.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def mr():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import mr_aha_amd
+    return mr_aha_amd
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def cfg_for(mr, g, **kw):
+    n = g.n[: g.n_spatial]
+    M = g.M[: g.n_spatial]
+    return mr.make_config(g.traj_type, n, g.L, phi2_dim=M, n_coils=g.n_coils, **kw)
+
+
+GOLDEN_CASES = ["2d_50x40_L3_c3", "2d_33x27_L1_c2_odd", "2d_50x40_L4_c2_M101x81",
+                "3d_24x20x16_L4_c2", "3d_25x21x15_L3_c2_odd", "stack_40x36x10_L3_c3",
+                "stack_31x36x9_L4_c2_odd", "checkop_2d", "checkop_stack", "radial_128_L4_C4"]
+
+
+def case(name):
+    from tests.golden.make_golden import case_inputs
+    return case_inputs(name)
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+@pytest.mark.parametrize("tri", ["lower", "upper"])
+def test_toeplitz_pipeline_vs_oracle(mr, golden, name, tri):
+    from oracle import toeplitz
+    print(f"[parity] {name} {tri}", flush=True)
+    gd = golden(name)
+    g, traj, phi2, x, S = case(name)
+    cfg = cfg_for(mr, g, triangulation=tri)
+    K = mr.calc_toep(cfg, dev(phi2), dev(traj))
+    torch.cuda.synchronize()
+    Kh = host(K)
+    assert Kh.shape == (g.T,) + g.grid_shape()
+    # kernel build vs exact Dirichlet-sum kernel
+    e_k = rel(Kh.reshape(g.T, -1)[:, gd["kidx"]], gd[f"K_{tri}"])
+    assert e_k < 1e-5, e_k
+    y = host(mr.aha_toeplitz(cfg, dev(x), dev(S), K))
+    # apply alone vs the FP64 five-stage apply on the same kernel
+    e_a = rel(y, toeplitz.apply(g, x, S, Kh.astype(np.complex128), upper=(tri == "upper")))
+    assert e_a < 1e-5, e_a
+    # end to end vs the exact Toeplitz and the exact NUDFT normal operator
+    assert rel(y, gd[f"AhAx_{tri}"]) < 1e-5
+    assert rel(y, gd["AhAx_nudft"]) < 1e-4
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_calc_mask_bit_exact(mr, name):
+    from oracle import mask as omask
+    g, traj, phi2, x, S = case(name)
+    cfg = cfg_for(mr, g)
+    m = host(mr.calc_mask(cfg, dev(traj)))
+    assert np.array_equal(m, omask.calc_mask(g, traj, cfg.mask_radius))
+
+
+@pytest.mark.parametrize("name", ["2d_50x40_L3_c3", "3d_25x21x15_L3_c2_odd", "stack_31x36x9_L4_c2_odd"])
+def test_masked_pipeline(mr, name):
+    from oracle import toeplitz
+    g, traj, phi2, x, S = case(name)
+    cfg = cfg_for(mr, g)
+    K = mr.calc_toep(cfg, dev(phi2), dev(traj))
+    cfgm = cfg_for(mr, g, is_masked=True)
+    mask = mr.calc_mask(cfgm, dev(traj))
+    Km = mr.calc_toep(cfgm, dev(phi2), dev(traj), mask)
+    assert cfgm.num_masked == mask.numel()
+    # masked kernel == unmasked gathered at the mask (same arithmetic)
+    assert np.array_equal(host(Km), host(K).reshape(g.T, -1)[:, host(mask)])
+    y = host(mr.aha_toeplitz(cfgm, dev(x), dev(S), Km, mask))
+    ref = toeplitz.apply(g, x, S, host(Km).astype(np.complex128), mask=host(mask))
+    assert rel(y, ref) < 1e-5
+
+
+@pytest.mark.parametrize("name", ["2d_50x40_L3_c3", "3d_24x20x16_L4_c2", "stack_40x36x10_L3_c3"])
+def test_operator_object(mr, name):
+    print(f"[op] {name}", flush=True)
+    g, traj, phi2, x, S = case(name)
+    dx, dS, dphi2, dtraj = dev(x), dev(S), dev(phi2), dev(traj)
+    for tri in ("lower", "upper"):
+        cfg = cfg_for(mr, g, triangulation=tri)
+        K = mr.calc_toep(cfg, dphi2, dtraj)
+        ref = mr.aha_toeplitz(cfg, dx, dS, K)
+        for share in (True, False):
+            with mr.create_op(cfg, dS, dphi2, dtraj, _phi2_toep=K, share_primary=share) as op:
+                y = op(dx)
+                assert torch.equal(y, ref), "injected kernel must be bit-identical"
+                assert op.info["toep_source"] == "supplied"
+        with mr.create_op(cfg, dS, dphi2, dtraj) as op:
+            assert rel(host(op.kernel(0)), host(K)) < 1e-6
+            y = op(dx)
+            assert rel(host(y), host(ref)) < 1e-6
+            inf = op.info
+            assert inf["devices"] == [torch.cuda.current_device()]
+            assert inf["toep_source"] == "broadcast" and inf["method"] == "toeplitz"
+            assert inf["coil_count"] == [g.n_coils] and inf["group"][0] >= 1
+            out = torch.empty_like(dx)
+            assert op.apply(dx, out=out) is out and torch.equal(out, y)
+
+
+def test_coil_group_invariance(mr, monkeypatch):
+    """Coil groups of 1, 2 and all coils give the same result (the reference's
+    coil-group invariance, tests/check_bd_kernels.py:13-17)."""
+    g, traj, phi2, x, S = case("stack_40x36x10_L3_c3")
+    cfg = cfg_for(mr, g)
+    K = mr.calc_toep(cfg, dev(phi2), dev(traj))
+    outs = []
+    for grp in ("1", "2", "3"):
+        monkeypatch.setenv("MR_AHA_HIP_COIL_GROUP", grp)
+        outs.append(host(mr.aha_toeplitz(cfg, dev(x), dev(S), K)))
+    assert rel(outs[0], outs[2]) < 1e-6 and rel(outs[1], outs[2]) < 1e-6
+
+
+@pytest.mark.parametrize("L", [1, 2, 5, 7, 12, 13, 16, 17, 20])
+def test_block_ranks(mr, L):
+    """Every rank (specialised kernels up to 16, the any-rank kernel above)
+    against the FP64 apply, both triangles, random packed kernel."""
+    from oracle import conventions as cv
+    from oracle import toeplitz
+    rng = np.random.default_rng(100 + L)
+    g = cv.make_geometry("2D", (12, 10), L, 3)
+    T = g.T
+    K = (rng.standard_normal((T,) + g.grid_shape()) + 1j * rng.standard_normal((T,) + g.grid_shape())).astype(np.complex64)
+    x = (rng.standard_normal((L,) + g.img_shape()) + 1j * rng.standard_normal((L,) + g.img_shape())).astype(np.complex64)
+    S = (rng.standard_normal((3,) + g.img_shape()) + 1j * rng.standard_normal((3,) + g.img_shape())).astype(np.complex64)
+    for tri in ("lower", "upper"):
+        cfg = cfg_for(mr, g, triangulation=tri)
+        y = host(mr.aha_toeplitz(cfg, dev(x), dev(S), dev(K)))
+        ref = toeplitz.apply(g, x, S, K.astype(np.complex128), upper=(tri == "upper"))
+        assert rel(y, ref) < 1e-5, (L, tri)
+
+
+def test_calc_toep_from_psf(mr):
+    """Two-stage build: the exact sign -1 point-spread functions finished on the
+    GPU equal the one-shot kernel."""
+    from oracle import conventions as cv
+    from oracle import toeplitz
+    g, traj, phi2, x, S = case("stack_31x36x9_L4_c2_odd")
+    cfg = cfg_for(mr, g)
+    K = toeplitz.calc_kernel(g, phi2, traj)
+    # psf_t[delta] = prod M_tr * (centered forward DFT of phi2_toep) restated:
+    Mx, My = g.M[0], g.M[1]
+    Kc = np.fft.ifftshift(K, axes=(-2, -1))
+    psf = np.fft.fftshift(np.fft.fft2(Kc), axes=(-2, -1))
+    out = torch.zeros((g.T,) + g.grid_shape(), dtype=torch.complex64, device="cuda")
+    for t0 in range(0, g.T, 3):
+        n = min(3, g.T - t0)
+        mr.calc_toep_from_psf(cfg, dev(psf[t0:t0 + n].astype(np.complex64)), t0, out)
+    assert rel(host(out), K) < 1e-5
+
+
+def test_argument_errors(mr, monkeypatch):
+    g, traj, phi2, x, S = case("checkop_2d")
+    dx, dS, dphi2, dtraj = dev(x), dev(S), dev(phi2), dev(traj)
+    cfg = cfg_for(mr, g)
+    with pytest.raises(ValueError, match="method"):
+        mr.create_op(cfg, dS, dphi2, dtraj, method="dft")
+    cfgm = cfg_for(mr, g, is_masked=True)
+    mask = mr.calc_mask(cfgm, dtraj)
+    with pytest.raises(ValueError, match="mask"):
+        mr.create_op(cfgm, dS, dphi2, dtraj, mask, method="nufft")
+    with pytest.raises(ValueError, match="mask"):
+        mr.create_op(cfgm, dS, dphi2, dtraj)
+    with pytest.raises(ValueError, match="nufft"):
+        mr.create_op(cfg, dS, dphi2, dtraj, nufft_opts={"upsampfac": 2})
+    with pytest.raises(ValueError, match="nufft_opts"):
+        mr.create_op(cfg, dS, dphi2, dtraj, method="nufft", nufft_opts={"spread": 1})
+    with pytest.raises(ValueError, match="devices"):
+        mr.create_op(cfg, dS, dphi2, dtraj, devices="some")
+    with pytest.raises(mr.MRError, match="not visible"):
+        mr.create_op(cfg, dS, dphi2, dtraj, devices=[99])
+    with pytest.raises(ValueError, match="toep_dist"):
+        mr.create_op(cfg, dS, dphi2, dtraj, toep_dist="ring")
+    with pytest.raises(ValueError, match="SEs"):
+        mr.create_op(cfg, dS[:1], dphi2, dtraj)
+    with pytest.raises(TypeError, match="create_op"):
+        mr.AhAOp(cfg)
+    with mr.create_op(cfg, dS, dphi2, dtraj, devices=np.int64(torch.cuda.current_device())) as op:
+        assert op.info["devices"] == [torch.cuda.current_device()]
+    with mr.create_op(cfg, dS, dphi2, dtraj, devices="all") as op:
+        assert len(op.info["devices"]) == 1
+    with mr.create_op(cfg, dS, dphi2, dtraj) as op:
+        L0 = cfg.L
+        cfg.L = L0 + 1
+        try:
+            assert op.config.L == L0 and op(dx).shape == dx.shape
+        finally:
+            cfg.L = L0
+    monkeypatch.setenv("MR_AHA_TOEP_SPLIT", "bogus")
+    with pytest.raises(mr.MRError, match="MR_AHA_TOEP_SPLIT"):
+        mr.create_op(cfg, dS, dphi2, dtraj)
+    monkeypatch.delenv("MR_AHA_TOEP_SPLIT")
+    op = mr.create_op(cfg, dS, dphi2, dtraj)
+    op.close()
+    op.close()
+    with pytest.raises(RuntimeError, match="closed"):
+        op(dx)
+    with pytest.raises(ValueError, match="x"):
+        mr.create_op(cfg, dS, dphi2, dtraj)(dx[:1])
+    with pytest.raises(TypeError, match="complex64"):
+        mr.aha_toeplitz(cfg, dx.to(torch.complex128), dS, dx)
+    assert torch.cuda.current_device() == 0
+
+
+def test_no_leak(mr):
+    g, traj, phi2, x, S = case("checkop_2d")
+    dx, dS, dphi2, dtraj = dev(x), dev(S), dev(phi2), dev(traj)
+    cfg = cfg_for(mr, g)
+    with mr.create_op(cfg, dS, dphi2, dtraj) as op:
+        op(dx)
+    torch.cuda.synchronize()
+    before = torch.cuda.mem_get_info()[0]
+    for _ in range(5):
+        with mr.create_op(cfg, dS, dphi2, dtraj) as op:
+            op(dx)
+    torch.cuda.synchronize()
+    after = torch.cuda.mem_get_info()[0]
+    assert before - after <= 8 << 20
