@@ -150,54 +150,50 @@ __device__ __forceinline__ void mr_bd_walk(const float2 (&v)[NC][L], float2 (&y)
 }
 
 // In place over K [P, L, ncoil] (natural order). Unmasked: kernel [P, T]
-// centered. Masked: inv[loc] is the kernel row of natural location loc or -1.
+// centered. Masked: inv[loc] is the kernel row of natural location loc or -1
+// (a location outside the mask multiplies by zero).
+//
+// One workgroup per tile of MR_BD_TP consecutive locations: the tile's T
+// packed kernel entries are staged once in LDS (coalesced: consecutive
+// natural locations are consecutive centered locations except at the one
+// wrap point of the x axis) and every coil of the group is multiplied from
+// LDS, so the kernel streams from HBM once per apply. Thread (p, slot)
+// handles location p for coils slot, slot + 8, ...
+#define MR_BD_TP 32
 template <int L, bool MASKED>
 __global__ void __launch_bounds__(256)
 mr_k_bd(float2 *__restrict__ K, const float2 *__restrict__ ker, const int *__restrict__ inv,
         long long kstride, mr_dims d, int ncoil)
 {
-    const long long loc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (loc >= d.pad_size) return;
+    constexpr int T = L * (L + 1) / 2;
+    __shared__ float2 kt[T * MR_BD_TP];
+    __shared__ long long kb[MR_BD_TP];
     const long long P = d.pad_size;
-    const long long slot = P * L;
-    const float2 *base;
-    if (MASKED) {
-        const int m = inv[loc];
-        if (m < 0) {
-            for (int c = 0; c < ncoil; ++c)
-#pragma unroll
-                for (int l = 0; l < L; ++l) K[loc + P * l + slot * c] = make_float2(0.f, 0.f);
-            return;
-        }
-        base = ker + m;
-    } else {
-        base = ker + mr_nat_to_cent_lin(loc, d);
+    const long long loc0 = mr_xcd_block(blockIdx.x, gridDim.x) * MR_BD_TP;
+    if (threadIdx.x < MR_BD_TP) {
+        const long long loc = loc0 + threadIdx.x;
+        long long b = -1;
+        if (loc < P) b = MASKED ? (long long)inv[loc] : mr_nat_to_cent_lin(loc, d);
+        kb[threadIdx.x] = b;
     }
-    int c = 0;
-    constexpr bool PAIR = (L <= 12);
-    if (PAIR) {
-        for (; c + 1 < ncoil; c += 2) {
-            float2 *g = K + loc + slot * c;
-            float2 v[2][L], y[2][L];
-#pragma unroll
-            for (int l = 0; l < L; ++l) {
-                v[0][l] = g[P * l];
-                v[1][l] = g[slot + P * l];
-            }
-            mr_bd_walk<L, 2>(v, y, base, kstride, d.tri_upper);
-#pragma unroll
-            for (int l = 0; l < L; ++l) {
-                g[P * l] = y[0][l];
-                g[slot + P * l] = y[1][l];
-            }
-        }
+    __syncthreads();
+    for (int f = threadIdx.x; f < T * MR_BD_TP; f += blockDim.x) {
+        const int t = f / MR_BD_TP, p = f - t * MR_BD_TP;
+        const long long b = kb[p];
+        kt[f] = (b >= 0) ? ker[b + kstride * t] : make_float2(0.f, 0.f);
     }
-    for (; c < ncoil; ++c) {
-        float2 *g = K + loc + slot * c;
+    __syncthreads();
+    const int p = threadIdx.x % MR_BD_TP;
+    const int slot = threadIdx.x / MR_BD_TP;
+    const int nslot = blockDim.x / MR_BD_TP;
+    const long long loc = loc0 + p;
+    if (loc >= P) return;
+    for (int c = slot; c < ncoil; c += nslot) {
+        float2 *g = K + loc + P * (long long)L * c;
         float2 v[1][L], y[1][L];
 #pragma unroll
         for (int l = 0; l < L; ++l) v[0][l] = g[P * l];
-        mr_bd_walk<L, 1>(v, y, base, kstride, d.tri_upper);
+        mr_bd_walk<L, 1>(v, y, kt + p, MR_BD_TP, d.tri_upper);
 #pragma unroll
         for (int l = 0; l < L; ++l) g[P * l] = y[0][l];
     }
@@ -486,7 +482,7 @@ static int mr_blockdiag(mr_aha_plan p, float2 *K, int nc, const float2 *ker)
     const long long kstride = masked ? (long long)p->cfg.num_masked : d.pad_size;
     mr_bd_fn fn = masked ? mr_bd_pick<true>(d.L) : mr_bd_pick<false>(d.L);
     if (fn) {
-        const long long blocks = mr_ceil_div(d.pad_size, 256);
+        const long long blocks = mr_ceil_div(d.pad_size, MR_BD_TP);
         hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(256), 0, p->stream, K, ker, p->d_inv,
                            kstride, d, nc);
         MR_LAUNCH_CHECK("mr_k_bd");
@@ -541,7 +537,7 @@ static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, cons
         {
             mr_ld_pad_sx ld{x, S, N, NyNz, Nx, Mx, d.c_n[0], L, coil0};
             mr_st_axis<float2, true> st{A, 1, Mx};
-            MR_TRY((mr_fft_pass<float, false>(p->fx, ld, st, NyNz * V, 0, 4, s, "F1")));
+            MR_TRY((mr_fft_pass<float, false>(p->fx, ld, st, NyNz * V, 0, 8, s, "F1")));
             MR_TRY(mr_prof_mark(p, MR_ST_F1));
         }
         // F2: y-FFT -> B [Mx, My, Nz, V]
@@ -556,7 +552,7 @@ static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, cons
             // F3: z-FFT -> A [Mx, My, Mz, V]
             mr_ld_pad<float2, false> ld{B, (long long)Mx * My, Mz, Nz, d.c_n[2]};
             mr_st_axis<float2, false> st{A, (long long)Mx * My, Mz};
-            MR_TRY((mr_fft_pass<float, false>(p->fz, ld, st, (long long)Mx * My * V, 0, 16, s, "F3")));
+            MR_TRY((mr_fft_pass<float, false>(p->fz, ld, st, (long long)Mx * My * V, 0, 64, s, "F3")));
             MR_TRY(mr_prof_mark(p, MR_ST_F3));
             K = A;
             free_buf = B;
@@ -568,7 +564,7 @@ static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, cons
             // I3: inverse z-FFT, keep Nz planes -> free_buf [Mx, My, Nz, V]
             mr_ld_axis<float2, false> ld{K, (long long)Mx * My, Mz};
             mr_st_crop<float2, false> st{free_buf, (long long)Mx * My, Mz, Nz, d.c_n[2]};
-            MR_TRY((mr_fft_pass<float, true>(p->fz, ld, st, (long long)Mx * My * V, 0, 16, s, "I3")));
+            MR_TRY((mr_fft_pass<float, true>(p->fz, ld, st, (long long)Mx * My * V, 0, 64, s, "I3")));
             MR_TRY(mr_prof_mark(p, MR_ST_I3));
             src = free_buf;
             free_buf = K;
@@ -585,7 +581,7 @@ static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, cons
         {
             mr_ld_row_coil ld{dst, NyNz, Mx, L, nc};
             mr_st_crop_conj_sum st{out, S, N, NyNz, Nx, Mx, d.c_n[0], coil0, coil0 > 0 ? 1 : 0};
-            MR_TRY((mr_fft_pass<float, true>(p->fx, ld, st, NyNz * L, (size_t)Nx, 4, s, "I1")));
+            MR_TRY((mr_fft_pass<float, true>(p->fx, ld, st, NyNz * L, (size_t)Nx, 8, s, "I1")));
             MR_TRY(mr_prof_mark(p, MR_ST_I1));
         }
     }

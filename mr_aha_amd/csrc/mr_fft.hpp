@@ -290,6 +290,16 @@ __device__ typename cvec<R>::T *mr_fft_lds(const mr_fft_len<R> &f, typename cvec
     return A;
 }
 
+// Hardware dispatches workgroup b of a 1-D grid to XCD b mod 8. Renumber so
+// that every XCD receives a contiguous run of logical tiles: neighbouring
+// tiles (which share cache lines and kernel rows) then meet in one L2.
+__device__ __forceinline__ long long mr_xcd_block(long long bid, long long nb)
+{
+    const long long q = nb / 8, r = nb % 8;
+    const long long x = bid % 8, i = bid / 8;
+    return x * q + (x < r ? x : r) + i;
+}
+
 // LDS pencil stride: odd for even n so that strided LDS accesses of
 // neighbouring pencils fall in different banks.
 __host__ __device__ inline int mr_fft_ns(int n) { return (n % 2 == 0) ? n + 1 : n; }
@@ -314,7 +324,7 @@ mr_fft_axis_kernel(mr_fft_len<R> f, Load ld, Store st, long long npencil, int TX
     V *A0 = tw + n;
     V *B0 = A0 + (size_t)TX * ns;
     V *acc = B0 + (size_t)TX * ns;  // Store scratch (TX * Store::acc_len)
-    const long long p0 = (long long)blockIdx.x * TX;
+    const long long p0 = mr_xcd_block(blockIdx.x, gridDim.x) * TX;
     const int tp = (int)((npencil - p0 < TX) ? (npencil - p0) : TX);
 
     for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = f.tw[i];
@@ -344,6 +354,116 @@ mr_fft_axis_kernel(mr_fft_len<R> f, Load ld, Store st, long long npencil, int TX
             for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
                 const int e = q / tp, p = q - e * tp;
                 st.put(p0 + p, p, e, sub, res[p * ns + e], acc);
+            }
+        }
+        __syncthreads();
+    }
+    st.flush(p0, tp, acc);
+}
+
+// ---------------------------------------------------------------------------
+// Register-staged variant for smooth lengths: one LDS buffer instead of two.
+// Every stage first reads all of a thread's butterfly inputs into registers,
+// then (after a barrier) writes the outputs back in place, so a workgroup
+// needs TX*n + n LDS values and can hold twice the pencils of the
+// ping-pong kernel. Requires TX*n <= MR_FFT_EMAX and radices in {2..8}.
+// ---------------------------------------------------------------------------
+
+#define MR_FFT_EMAX 6144
+#define MR_FFT_FAST_THREADS 512
+
+template <int RR, bool INV, typename V>
+__device__ __forceinline__ void mr_stage_reg(V *A, const V *tw, int n, int ns, int tp, int Ns)
+{
+    constexpr int NT = MR_FFT_FAST_THREADS;
+    constexpr int KMAX = (MR_FFT_EMAX / RR + NT - 1) / NT;
+    const int m = n / RR;
+    const int stride = n / (Ns * RR);
+    const int nb = tp * m;
+    V v[KMAX][RR];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        const int f = threadIdx.x + k * NT;
+        if (f < nb) {
+            const int p = f / m, j = f - p * m;
+            const V *a = A + p * ns + j;
+#pragma unroll
+            for (int r = 0; r < RR; ++r) v[k][r] = a[r * m];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        const int f = threadIdx.x + k * NT;
+        if (f < nb) {
+            const int p = f / m, j = f - p * m;
+            const int kk = j % Ns;
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < RR; ++r) {
+                    const V w = tw[kk * r * stride];
+                    v[k][r] = INV ? cmulc(w, v[k][r]) : cmul(w, v[k][r]);
+                }
+            }
+            bfly<RR, INV>(v[k]);
+            V *b = A + p * ns + (j / Ns) * Ns * RR + kk;
+#pragma unroll
+            for (int q = 0; q < RR; ++q) b[q * Ns] = v[k][q];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename R, bool INV, class Load, class Store>
+__global__ void __launch_bounds__(MR_FFT_FAST_THREADS)
+mr_fft_axis_fast(mr_fft_len<R> f, Load ld, Store st, long long npencil, int TX)
+{
+    using V = typename cvec<R>::T;
+    extern __shared__ __align__(16) unsigned char mr_smem[];
+    const int n = f.n;
+    const int ns = mr_fft_ns(n);
+    V *tw = reinterpret_cast<V *>(mr_smem);
+    V *A = tw + n;
+    V *acc = A + (size_t)TX * ns;
+    const long long p0 = mr_xcd_block(blockIdx.x, gridDim.x) * TX;
+    const int tp = (int)((npencil - p0 < TX) ? (npencil - p0) : TX);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = f.tw[i];
+    const int nsub = ld.nsub();
+    for (int sub = 0; sub < nsub; ++sub) {
+        if (Load::kElemFast) {
+            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
+                const int p = q / n, e = q - p * n;
+                A[p * ns + e] = ld(p0 + p, e, sub);
+            }
+        } else {
+            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
+                const int e = q / tp, p = q - e * tp;
+                A[p * ns + e] = ld(p0 + p, e, sub);
+            }
+        }
+        __syncthreads();
+        int Ns = 1;
+        for (int s = 0; s < f.nst; ++s) {
+            const int RR = f.radix[s];
+            switch (RR) {
+            case 2: mr_stage_reg<2, INV, V>(A, tw, n, ns, tp, Ns); break;
+            case 3: mr_stage_reg<3, INV, V>(A, tw, n, ns, tp, Ns); break;
+            case 4: mr_stage_reg<4, INV, V>(A, tw, n, ns, tp, Ns); break;
+            case 5: mr_stage_reg<5, INV, V>(A, tw, n, ns, tp, Ns); break;
+            case 7: mr_stage_reg<7, INV, V>(A, tw, n, ns, tp, Ns); break;
+            default: mr_stage_reg<8, INV, V>(A, tw, n, ns, tp, Ns); break;
+            }
+            Ns *= RR;
+        }
+        if (Store::kElemFast) {
+            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
+                const int p = q / n, e = q - p * n;
+                st.put(p0 + p, p, e, sub, A[p * ns + e], acc);
+            }
+        } else {
+            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
+                const int e = q / tp, p = q - e * tp;
+                st.put(p0 + p, p, e, sub, A[p * ns + e], acc);
             }
         }
         __syncthreads();

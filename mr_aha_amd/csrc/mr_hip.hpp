@@ -30,12 +30,34 @@ void mr_pop_device(int prev);
 // Budget = clamp(mem_fraction) * free bytes of the current device.
 int mr_mem_budget(const mr_config *cfg, size_t *budget);
 
-// Launch one axis pass of the FFT engine on `stream`.
+// Launch one axis pass of the FFT engine on `stream`: the register-staged
+// single-buffer kernel for smooth FP32 lengths, the ping-pong kernel
+// otherwise (any length, FP64).
 template <typename R, bool INV, class Load, class Store>
 int mr_fft_pass(const mr_fft_len<R> &f, const Load &ld, const Store &st, long long npencil,
                 size_t acc_per_pencil, int want_tx, hipStream_t stream, const char *name)
 {
+    using V = typename cvec<R>::T;
     if (npencil <= 0) return MR_SUCCESS;
+    static const bool fast_on = mr_env_ll("MR_AHA_HIP_FFT_FAST", 0) != 0;
+    bool smooth = sizeof(R) == 4 && fast_on;
+    for (int i = 0; i < f.nst && smooth; ++i) smooth = (f.radix[i] <= 8 && f.radix[i] != 6);
+    if (smooth && f.n <= MR_FFT_EMAX) {
+        const int ns = mr_fft_ns(f.n);
+        int TX = want_tx;
+        auto need = [&](int tx) {
+            return ((size_t)f.n + (size_t)tx * ns + (size_t)tx * acc_per_pencil) * sizeof(V);
+        };
+        while (TX > 1 && ((long long)TX * f.n > MR_FFT_EMAX || need(TX) > 64 * 1024)) TX /= 2;
+        if (TX > npencil) TX = (int)npencil;
+        const long long blocks = (npencil + TX - 1) / TX;
+        if (blocks > 0x7fffffffLL)
+            MR_FAIL(MR_ERR_UNSUPPORTED, "%s: %lld pencils exceed the launch limit", name, npencil);
+        hipLaunchKernelGGL((mr_fft_axis_fast<R, INV, Load, Store>), dim3((unsigned)blocks),
+                           dim3(MR_FFT_FAST_THREADS), need(TX), stream, f, ld, st, npencil, TX);
+        MR_LAUNCH_CHECK(name);
+        return MR_SUCCESS;
+    }
     mr_fft_launch l = mr_fft_pick<R>(f.n, npencil, acc_per_pencil, want_tx);
     if (l.blocks > 0x7fffffffLL)
         MR_FAIL(MR_ERR_UNSUPPORTED, "%s: %lld pencils exceed the launch limit", name, npencil);

@@ -230,8 +230,9 @@ def test_argument_errors(mr, monkeypatch):
         op(dx)
     with pytest.raises(ValueError, match="x"):
         mr.create_op(cfg, dS, dphi2, dtraj)(dx[:1])
+    K = mr.calc_toep(cfg, dphi2, dtraj)
     with pytest.raises(TypeError, match="complex64"):
-        mr.aha_toeplitz(cfg, dx.to(torch.complex128), dS, dx)
+        mr.aha_toeplitz(cfg, dx.to(torch.complex128), dS, K)
     assert torch.cuda.current_device() == 0
 
 
