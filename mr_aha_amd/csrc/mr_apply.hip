@@ -35,62 +35,73 @@
 // F1 load: pencil p = yz + NyNz * v, v = l + L * c; natural element e.
 struct mr_ld_pad_sx {
     static constexpr bool kElemFast = true;
+    struct PB {
+        long long xo, so;
+    };
     const float2 *x, *S;
     long long N, NyNz;
     int Nx, Mx, cnx, L, coil0;
     __device__ int nsub() const { return 1; }
-    __device__ float2 operator()(long long p, int e, int) const
+    __device__ PB pencil(long long p, int) const
+    {
+        const long long v = p / NyNz, yz = p - v * NyNz;
+        const int l = (int)(v % L), c = (int)(v / L);
+        return PB{(long long)Nx * yz + N * l, (long long)Nx * yz + N * (coil0 + c)};
+    }
+    __device__ float2 get(const PB &b, int e) const
     {
         int i = e + cnx;
         if (i >= Mx) i -= Mx;
-        if (i >= Nx) return make_float2(0.f, 0.f);
-        const long long v = p / NyNz, yz = p - v * NyNz;
-        const int l = (int)(v % L), c = (int)(v / L);
-        const long long ii = i + (long long)Nx * yz;
-        return cmul(S[ii + N * (coil0 + c)], x[ii + N * l]);
+        return (i < Nx) ? cmul(S[b.so + i], x[b.xo + i]) : make_float2(0.f, 0.f);
     }
 };
 
 // I1 load: pencil p = yz + NyNz * l, round `sub` = coil of the group.
 struct mr_ld_row_coil {
     static constexpr bool kElemFast = true;
+    using PB = long long;
     const float2 *in;
     long long NyNz;
     int Mx, L, ncoil;
     __device__ int nsub() const { return ncoil; }
-    __device__ float2 operator()(long long p, int e, int sub) const
+    __device__ PB pencil(long long p, int sub) const
     {
-        return in[e + (long long)Mx * (p + NyNz * (long long)L * sub)];
+        return (long long)Mx * (p + NyNz * (long long)L * sub);
     }
+    __device__ float2 get(const PB &b, int e) const { return in[b + e]; }
 };
 
 // I1 store: crop, conj(S_c) and coil sum in LDS; flush writes AhAx.
 struct mr_st_crop_conj_sum {
     static constexpr bool kElemFast = true;
+    using PB = long long;  // offset of the coil's map row
     float2 *out;
     const float2 *S;
     long long N, NyNz;
     int Nx, Mx, cnx, coil0, accumulate;
-    __device__ void put(long long p, int pl, int e, int sub, float2 v, float2 *acc) const
+    __device__ PB pencil(long long p, int sub) const
+    {
+        const long long yz = p % NyNz;
+        return (long long)Nx * yz + N * (coil0 + sub);
+    }
+    __device__ void put(const PB &b, int pl, int e, int sub, float2 v, float2 *acc) const
     {
         int i = e + cnx;
         if (i >= Mx) i -= Mx;
         if (i >= Nx) return;
-        const long long yz = p % NyNz;
-        const float2 s = S[i + (long long)Nx * yz + N * (coil0 + sub)];
-        const float2 t = cmulc(s, v);
+        const float2 t = cmulc(S[b + i], v);
         float2 *a = acc + (size_t)pl * Nx + i;
         *a = (sub == 0) ? t : cadd(*a, t);
     }
     __device__ void flush(long long p0, int tp, float2 *acc) const
     {
-        for (int q = threadIdx.x; q < tp * Nx; q += blockDim.x) {
-            const int pl = q / Nx, i = q - pl * Nx;
+        for (int pl = 0; pl < tp; ++pl) {
             const long long p = p0 + pl;
             const long long l = p / NyNz, yz = p - l * NyNz;
-            const long long o = i + (long long)Nx * yz + N * l;
-            const float2 a = acc[q];
-            out[o] = accumulate ? cadd(out[o], a) : a;
+            float2 *o = out + (long long)Nx * yz + N * l;
+            const float2 *a = acc + (size_t)pl * Nx;
+            for (int i = threadIdx.x; i < Nx; i += blockDim.x)
+                o[i] = accumulate ? cadd(o[i], a[i]) : a[i];
         }
     }
 };

@@ -51,13 +51,58 @@ template <bool INV, typename V> __device__ __forceinline__ V cmul_mi(V a)
     return INV ? cmake<V>(-a.y, a.x) : cmake<V>(a.y, -a.x);
 }
 
+// Division by a run-time constant d as a multiply-high and a shift
+// (round-up multiplier m = ceil(2^(31+l)/d), l = ceil(log2 d)); exact for
+// dividends below 2^31. Built on the host, used in index maps on the device.
+struct mr_fdiv {
+    unsigned d, mul, shr;
+};
+inline mr_fdiv mr_fdiv_make(unsigned d)
+{
+    mr_fdiv f;
+    f.d = d;
+    if (d <= 1) {
+        f.mul = 0;
+        f.shr = 0;
+        return f;
+    }
+    unsigned l = 0;
+    while ((1u << l) < d) ++l;
+    const unsigned p = 31 + l;
+    f.mul = (unsigned)(((1ull << p) + d - 1) / d);
+    f.shr = p - 32;
+    return f;
+}
+__host__ __device__ __forceinline__ unsigned mr_fdiv_q(const mr_fdiv &f, unsigned n)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return (f.d <= 1) ? n : (__umulhi(n, f.mul) >> f.shr);
+#else
+    return (f.d <= 1) ? n : (unsigned)(((unsigned long long)n * f.mul) >> 32) >> f.shr;
+#endif
+}
+
 // Per-length plan passed by value to kernels.
 template <typename R> struct mr_fft_len {
     int n;
     int nst;
     int radix[MR_FFT_MAXST];
+    mr_fdiv mdiv[MR_FFT_MAXST];    // n / radix[s]
+    mr_fdiv nsdiv[MR_FFT_MAXST];   // product of the radices before stage s
+    mr_fdiv ndiv;                  // n
     const typename cvec<R>::T *tw;  // tw[q] = exp(-2 pi i q / n), q < n (device)
 };
+
+template <typename R> inline void mr_fft_fill_divs(mr_fft_len<R> *f)
+{
+    int Ns = 1;
+    for (int s = 0; s < f->nst; ++s) {
+        f->mdiv[s] = mr_fdiv_make((unsigned)(f->n / f->radix[s]));
+        f->nsdiv[s] = mr_fdiv_make((unsigned)Ns);
+        Ns *= f->radix[s];
+    }
+    f->ndiv = mr_fdiv_make((unsigned)f->n);
+}
 
 // Factor n into the stage radices: 8s first, then 4, 2, 3, 5, 7, then the
 // remaining primes in ascending order.
@@ -211,14 +256,16 @@ template <int RR, bool INV, typename V> __device__ __forceinline__ void bfly(V *
 // (pencil stride ns) of length n, Ns = product of earlier radices.
 template <int RR, bool INV, typename V>
 __device__ __forceinline__ void mr_stage(const V *__restrict__ A, V *__restrict__ B, const V *tw,
-                                         int n, int ns, int tp, int Ns)
+                                         int n, int ns, int tp, int Ns, const mr_fdiv &mdv,
+                                         const mr_fdiv &nsdv)
 {
     const int m = n / RR;
     const int stride = n / (Ns * RR);
     for (int f = threadIdx.x; f < tp * m; f += blockDim.x) {
-        const int p = f / m;
+        const int p = (int)mr_fdiv_q(mdv, (unsigned)f);
         const int j = f - p * m;
-        const int k = j % Ns;
+        const int jn = (int)mr_fdiv_q(nsdv, (unsigned)j);
+        const int k = j - jn * Ns;
         const V *a = A + p * ns + j;
         V v[RR];
 #pragma unroll
@@ -231,7 +278,7 @@ __device__ __forceinline__ void mr_stage(const V *__restrict__ A, V *__restrict_
             }
         }
         bfly<RR, INV>(v);
-        V *b = B + p * ns + (j / Ns) * Ns * RR + k;
+        V *b = B + p * ns + jn * Ns * RR + k;
 #pragma unroll
         for (int q = 0; q < RR; ++q) b[q * Ns] = v[q];
     }
@@ -274,13 +321,14 @@ __device__ typename cvec<R>::T *mr_fft_lds(const mr_fft_len<R> &f, typename cvec
     int Ns = 1;
     for (int s = 0; s < f.nst; ++s) {
         const int RR = f.radix[s];
+        const mr_fdiv &md = f.mdiv[s], &nd = f.nsdiv[s];
         switch (RR) {
-        case 2: mr_stage<2, INV, V>(A, B, tw, f.n, ns, tp, Ns); break;
-        case 3: mr_stage<3, INV, V>(A, B, tw, f.n, ns, tp, Ns); break;
-        case 4: mr_stage<4, INV, V>(A, B, tw, f.n, ns, tp, Ns); break;
-        case 5: mr_stage<5, INV, V>(A, B, tw, f.n, ns, tp, Ns); break;
-        case 7: mr_stage<7, INV, V>(A, B, tw, f.n, ns, tp, Ns); break;
-        case 8: mr_stage<8, INV, V>(A, B, tw, f.n, ns, tp, Ns); break;
+        case 2: mr_stage<2, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
+        case 3: mr_stage<3, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
+        case 4: mr_stage<4, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
+        case 5: mr_stage<5, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
+        case 7: mr_stage<7, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
+        case 8: mr_stage<8, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
         default: mr_stage_any<INV, V>(A, B, tw, f.n, ns, tp, Ns, RR); break;
         }
         __syncthreads();
@@ -304,58 +352,91 @@ __device__ __forceinline__ long long mr_xcd_block(long long bid, long long nb)
 // neighbouring pencils fall in different banks.
 __host__ __device__ inline int mr_fft_ns(int n) { return (n % 2 == 0) ? n + 1 : n; }
 
-// Generic axis kernel. Load::kElemFast selects the global access order
-// (true: consecutive threads walk one pencil; false: consecutive threads
-// walk consecutive pencils at one element). Load/Store see the global pencil
-// index and the element index in natural order.
+// Functor protocol of the axis kernels.
 //
-// Loads may run in several rounds (Load::nsub > 1, e.g. one per coil); each
-// round is transformed and handed to Store::put(p, e, sub, value), then
-// Store::flush(p, local_p, tp) runs once after the last round.
+// Load:  static constexpr bool kElemFast;  (true: a pencil's elements are
+//        contiguous in memory, so consecutive threads walk one pencil;
+//        false: consecutive threads walk consecutive pencils)
+//        struct PB;                         per-pencil base, computed once
+//        int nsub() const;                  load rounds (e.g. one per coil)
+//        PB pencil(long long p, int sub) const;
+//        V get(const PB &, int e) const;    natural element e of the pencil
+// Store: static constexpr bool kElemFast; struct PB;
+//        PB pencil(long long p, int sub) const;
+//        void put(const PB &, int pl, int e, int sub, V v, V *acc) const;
+//        void flush(long long p0, int tp, V *acc) const;   after the last round
+// Pencil bases live in LDS, so the per-element address is one multiply-add.
+#define MR_FFT_MAXTX 64
+
+template <class Load, class Store, typename V>
+__device__ __forceinline__ void mr_fft_load_tile(const Load &ld, typename Load::PB *lpb, V *A,
+                                                 const mr_fdiv &ndiv, int n, int ns, int tp,
+                                                 int lgTX)
+{
+    if (Load::kElemFast) {
+        for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
+            const int p = (int)mr_fdiv_q(ndiv, (unsigned)q), e = q - p * n;
+            A[p * ns + e] = ld.get(lpb[p], e);
+        }
+    } else {
+        const int TX = 1 << lgTX;
+        for (int q = threadIdx.x; q < (n << lgTX); q += blockDim.x) {
+            const int e = q >> lgTX, p = q & (TX - 1);
+            if (p < tp) A[p * ns + e] = ld.get(lpb[p], e);
+        }
+    }
+}
+
+template <class Store, typename V>
+__device__ __forceinline__ void mr_fft_store_tile(const Store &st, typename Store::PB *spb,
+                                                  const V *A, V *acc, const mr_fdiv &ndiv, int n,
+                                                  int ns, int tp, int lgTX, int sub)
+{
+    if (Store::kElemFast) {
+        for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
+            const int p = (int)mr_fdiv_q(ndiv, (unsigned)q), e = q - p * n;
+            st.put(spb[p], p, e, sub, A[p * ns + e], acc);
+        }
+    } else {
+        const int TX = 1 << lgTX;
+        for (int q = threadIdx.x; q < (n << lgTX); q += blockDim.x) {
+            const int e = q >> lgTX, p = q & (TX - 1);
+            if (p < tp) st.put(spb[p], p, e, sub, A[p * ns + e], acc);
+        }
+    }
+}
+
+// Ping-pong kernel: any length and precision. TX = 2^lgTX pencils per
+// workgroup (the last one may hold fewer).
 template <typename R, bool INV, class Load, class Store>
 __global__ void __launch_bounds__(256)
-mr_fft_axis_kernel(mr_fft_len<R> f, Load ld, Store st, long long npencil, int TX)
+mr_fft_axis_kernel(mr_fft_len<R> f, Load ld, Store st, long long npencil, int lgTX)
 {
     using V = typename cvec<R>::T;
     extern __shared__ __align__(16) unsigned char mr_smem[];
+    __shared__ typename Load::PB lpb[MR_FFT_MAXTX];
+    __shared__ typename Store::PB spb[MR_FFT_MAXTX];
+    const int TX = 1 << lgTX;
     const int n = f.n;
     const int ns = mr_fft_ns(n);
     V *tw = reinterpret_cast<V *>(mr_smem);
     V *A0 = tw + n;
     V *B0 = A0 + (size_t)TX * ns;
-    V *acc = B0 + (size_t)TX * ns;  // Store scratch (TX * Store::acc_len)
+    V *acc = B0 + (size_t)TX * ns;
     const long long p0 = mr_xcd_block(blockIdx.x, gridDim.x) * TX;
     const int tp = (int)((npencil - p0 < TX) ? (npencil - p0) : TX);
-
     for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = f.tw[i];
-
     const int nsub = ld.nsub();
     for (int sub = 0; sub < nsub; ++sub) {
-        V *A = A0, *B = B0;
-        if (Load::kElemFast) {
-            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
-                const int p = q / n, e = q - p * n;
-                A[p * ns + e] = ld(p0 + p, e, sub);
-            }
-        } else {
-            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
-                const int e = q / tp, p = q - e * tp;
-                A[p * ns + e] = ld(p0 + p, e, sub);
-            }
+        if (threadIdx.x < tp) {
+            lpb[threadIdx.x] = ld.pencil(p0 + threadIdx.x, sub);
+            spb[threadIdx.x] = st.pencil(p0 + threadIdx.x, sub);
         }
         __syncthreads();
-        V *res = mr_fft_lds<INV, R>(f, A, B, tw, ns, tp);
-        if (Store::kElemFast) {
-            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
-                const int p = q / n, e = q - p * n;
-                st.put(p0 + p, p, e, sub, res[p * ns + e], acc);
-            }
-        } else {
-            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
-                const int e = q / tp, p = q - e * tp;
-                st.put(p0 + p, p, e, sub, res[p * ns + e], acc);
-            }
-        }
+        mr_fft_load_tile<Load, Store, V>(ld, lpb, A0, f.ndiv, n, ns, tp, lgTX);
+        __syncthreads();
+        V *res = mr_fft_lds<INV, R>(f, A0, B0, tw, ns, tp);
+        mr_fft_store_tile<Store, V>(st, spb, res, acc, f.ndiv, n, ns, tp, lgTX, sub);
         __syncthreads();
     }
     st.flush(p0, tp, acc);
@@ -373,7 +454,8 @@ mr_fft_axis_kernel(mr_fft_len<R> f, Load ld, Store st, long long npencil, int TX
 #define MR_FFT_FAST_THREADS 512
 
 template <int RR, bool INV, typename V>
-__device__ __forceinline__ void mr_stage_reg(V *A, const V *tw, int n, int ns, int tp, int Ns)
+__device__ __forceinline__ void mr_stage_reg(V *A, const V *tw, int n, int ns, int tp, int Ns,
+                                             const mr_fdiv &mdv, const mr_fdiv &nsdv)
 {
     constexpr int NT = MR_FFT_FAST_THREADS;
     constexpr int KMAX = (MR_FFT_EMAX / RR + NT - 1) / NT;
@@ -385,7 +467,7 @@ __device__ __forceinline__ void mr_stage_reg(V *A, const V *tw, int n, int ns, i
     for (int k = 0; k < KMAX; ++k) {
         const int f = threadIdx.x + k * NT;
         if (f < nb) {
-            const int p = f / m, j = f - p * m;
+            const int p = (int)mr_fdiv_q(mdv, (unsigned)f), j = f - p * m;
             const V *a = A + p * ns + j;
 #pragma unroll
             for (int r = 0; r < RR; ++r) v[k][r] = a[r * m];
@@ -396,8 +478,9 @@ __device__ __forceinline__ void mr_stage_reg(V *A, const V *tw, int n, int ns, i
     for (int k = 0; k < KMAX; ++k) {
         const int f = threadIdx.x + k * NT;
         if (f < nb) {
-            const int p = f / m, j = f - p * m;
-            const int kk = j % Ns;
+            const int p = (int)mr_fdiv_q(mdv, (unsigned)f), j = f - p * m;
+            const int jn = (int)mr_fdiv_q(nsdv, (unsigned)j);
+            const int kk = j - jn * Ns;
             if (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < RR; ++r) {
@@ -406,7 +489,7 @@ __device__ __forceinline__ void mr_stage_reg(V *A, const V *tw, int n, int ns, i
                 }
             }
             bfly<RR, INV>(v[k]);
-            V *b = A + p * ns + (j / Ns) * Ns * RR + kk;
+            V *b = A + p * ns + jn * Ns * RR + kk;
 #pragma unroll
             for (int q = 0; q < RR; ++q) b[q * Ns] = v[k][q];
         }
@@ -416,10 +499,13 @@ __device__ __forceinline__ void mr_stage_reg(V *A, const V *tw, int n, int ns, i
 
 template <typename R, bool INV, class Load, class Store>
 __global__ void __launch_bounds__(MR_FFT_FAST_THREADS)
-mr_fft_axis_fast(mr_fft_len<R> f, Load ld, Store st, long long npencil, int TX)
+mr_fft_axis_fast(mr_fft_len<R> f, Load ld, Store st, long long npencil, int lgTX)
 {
     using V = typename cvec<R>::T;
     extern __shared__ __align__(16) unsigned char mr_smem[];
+    __shared__ typename Load::PB lpb[MR_FFT_MAXTX];
+    __shared__ typename Store::PB spb[MR_FFT_MAXTX];
+    const int TX = 1 << lgTX;
     const int n = f.n;
     const int ns = mr_fft_ns(n);
     V *tw = reinterpret_cast<V *>(mr_smem);
@@ -430,42 +516,28 @@ mr_fft_axis_fast(mr_fft_len<R> f, Load ld, Store st, long long npencil, int TX)
     for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = f.tw[i];
     const int nsub = ld.nsub();
     for (int sub = 0; sub < nsub; ++sub) {
-        if (Load::kElemFast) {
-            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
-                const int p = q / n, e = q - p * n;
-                A[p * ns + e] = ld(p0 + p, e, sub);
-            }
-        } else {
-            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
-                const int e = q / tp, p = q - e * tp;
-                A[p * ns + e] = ld(p0 + p, e, sub);
-            }
+        if (threadIdx.x < tp) {
+            lpb[threadIdx.x] = ld.pencil(p0 + threadIdx.x, sub);
+            spb[threadIdx.x] = st.pencil(p0 + threadIdx.x, sub);
         }
+        __syncthreads();
+        mr_fft_load_tile<Load, Store, V>(ld, lpb, A, f.ndiv, n, ns, tp, lgTX);
         __syncthreads();
         int Ns = 1;
         for (int s = 0; s < f.nst; ++s) {
             const int RR = f.radix[s];
+            const mr_fdiv &md = f.mdiv[s], &nd = f.nsdiv[s];
             switch (RR) {
-            case 2: mr_stage_reg<2, INV, V>(A, tw, n, ns, tp, Ns); break;
-            case 3: mr_stage_reg<3, INV, V>(A, tw, n, ns, tp, Ns); break;
-            case 4: mr_stage_reg<4, INV, V>(A, tw, n, ns, tp, Ns); break;
-            case 5: mr_stage_reg<5, INV, V>(A, tw, n, ns, tp, Ns); break;
-            case 7: mr_stage_reg<7, INV, V>(A, tw, n, ns, tp, Ns); break;
-            default: mr_stage_reg<8, INV, V>(A, tw, n, ns, tp, Ns); break;
+            case 2: mr_stage_reg<2, INV, V>(A, tw, n, ns, tp, Ns, md, nd); break;
+            case 3: mr_stage_reg<3, INV, V>(A, tw, n, ns, tp, Ns, md, nd); break;
+            case 4: mr_stage_reg<4, INV, V>(A, tw, n, ns, tp, Ns, md, nd); break;
+            case 5: mr_stage_reg<5, INV, V>(A, tw, n, ns, tp, Ns, md, nd); break;
+            case 7: mr_stage_reg<7, INV, V>(A, tw, n, ns, tp, Ns, md, nd); break;
+            default: mr_stage_reg<8, INV, V>(A, tw, n, ns, tp, Ns, md, nd); break;
             }
             Ns *= RR;
         }
-        if (Store::kElemFast) {
-            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
-                const int p = q / n, e = q - p * n;
-                st.put(p0 + p, p, e, sub, A[p * ns + e], acc);
-            }
-        } else {
-            for (int q = threadIdx.x; q < tp * n; q += blockDim.x) {
-                const int e = q / tp, p = q - e * tp;
-                st.put(p0 + p, p, e, sub, A[p * ns + e], acc);
-            }
-        }
+        mr_fft_store_tile<Store, V>(st, spb, A, acc, f.ndiv, n, ns, tp, lgTX, sub);
         __syncthreads();
     }
     st.flush(p0, tp, acc);
@@ -491,14 +563,14 @@ inline mr_fft_launch mr_fft_pick(int n, long long npencil, size_t acc_elems_per_
                                  int want_tx)
 {
     using V = typename cvec<R>::T;
-    const size_t budget = 96 * 1024;
+    const size_t budget = 64 * 1024;
     const int ns = mr_fft_ns(n);
     int TX = want_tx;
     auto need = [&](int tx) {
         return ((size_t)n + 2 * (size_t)tx * ns + (size_t)tx * acc_elems_per_pencil) * sizeof(V);
     };
     while (TX > 1 && need(TX) > budget) TX /= 2;
-    if (TX > npencil) TX = (int)(npencil < 1 ? 1 : npencil);
+    while (TX > 1 && TX / 2 >= npencil) TX /= 2;
     mr_fft_launch l;
     l.TX = TX;
     l.lds = need(TX);

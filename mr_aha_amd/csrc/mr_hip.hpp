@@ -31,57 +31,67 @@ void mr_pop_device(int prev);
 int mr_mem_budget(const mr_config *cfg, size_t *budget);
 
 // Launch one axis pass of the FFT engine on `stream`: the register-staged
-// single-buffer kernel for smooth FP32 lengths, the ping-pong kernel
-// otherwise (any length, FP64).
+// single-buffer kernel for smooth FP32 lengths when MR_AHA_HIP_FFT_FAST=1,
+// the ping-pong kernel otherwise (any length, FP64). TX is a power of two.
 template <typename R, bool INV, class Load, class Store>
 int mr_fft_pass(const mr_fft_len<R> &f, const Load &ld, const Store &st, long long npencil,
                 size_t acc_per_pencil, int want_tx, hipStream_t stream, const char *name)
 {
     using V = typename cvec<R>::T;
     if (npencil <= 0) return MR_SUCCESS;
+    if (want_tx > MR_FFT_MAXTX) want_tx = MR_FFT_MAXTX;
     static const bool fast_on = mr_env_ll("MR_AHA_HIP_FFT_FAST", 0) != 0;
     bool smooth = sizeof(R) == 4 && fast_on;
     for (int i = 0; i < f.nst && smooth; ++i) smooth = (f.radix[i] <= 8 && f.radix[i] != 6);
+    int TX;
+    size_t lds;
     if (smooth && f.n <= MR_FFT_EMAX) {
         const int ns = mr_fft_ns(f.n);
-        int TX = want_tx;
         auto need = [&](int tx) {
             return ((size_t)f.n + (size_t)tx * ns + (size_t)tx * acc_per_pencil) * sizeof(V);
         };
+        TX = want_tx;
         while (TX > 1 && ((long long)TX * f.n > MR_FFT_EMAX || need(TX) > 64 * 1024)) TX /= 2;
-        if (TX > npencil) TX = (int)npencil;
-        const long long blocks = (npencil + TX - 1) / TX;
-        if (blocks > 0x7fffffffLL)
-            MR_FAIL(MR_ERR_UNSUPPORTED, "%s: %lld pencils exceed the launch limit", name, npencil);
-        hipLaunchKernelGGL((mr_fft_axis_fast<R, INV, Load, Store>), dim3((unsigned)blocks),
-                           dim3(MR_FFT_FAST_THREADS), need(TX), stream, f, ld, st, npencil, TX);
-        MR_LAUNCH_CHECK(name);
-        return MR_SUCCESS;
+        while (TX > 1 && TX / 2 >= npencil) TX /= 2;
+        lds = need(TX);
+    } else {
+        smooth = false;
+        mr_fft_launch l = mr_fft_pick<R>(f.n, npencil, acc_per_pencil, want_tx);
+        TX = l.TX;
+        lds = l.lds;
     }
-    mr_fft_launch l = mr_fft_pick<R>(f.n, npencil, acc_per_pencil, want_tx);
-    if (l.blocks > 0x7fffffffLL)
+    int lg = 0;
+    while ((1 << lg) < TX) ++lg;
+    const long long blocks = (npencil + TX - 1) / TX;
+    if (blocks > 0x7fffffffLL)
         MR_FAIL(MR_ERR_UNSUPPORTED, "%s: %lld pencils exceed the launch limit", name, npencil);
-    hipLaunchKernelGGL((mr_fft_axis_kernel<R, INV, Load, Store>), dim3((unsigned)l.blocks),
-                       dim3(256), l.lds, stream, f, ld, st, npencil, l.TX);
+    if (smooth)
+        hipLaunchKernelGGL((mr_fft_axis_fast<R, INV, Load, Store>), dim3((unsigned)blocks),
+                           dim3(MR_FFT_FAST_THREADS), lds, stream, f, ld, st, npencil, lg);
+    else
+        hipLaunchKernelGGL((mr_fft_axis_kernel<R, INV, Load, Store>), dim3((unsigned)blocks),
+                           dim3(256), lds, stream, f, ld, st, npencil, lg);
     MR_LAUNCH_CHECK(name);
     return MR_SUCCESS;
 }
 
-// Generic axis functors over an array viewed as [I, len, O] (inner stride 1).
-//   pencil p = i + I * o ; element e along the axis.
+// Generic axis functors over an array viewed as [I, len, O] (inner stride 1):
+// pencil p = i + I * o, element e along the axis at offset i + I*(e + len*o).
 
-// Full-length load, element e at offset i + I*(e + n*o).
+// Full-length load.
 template <typename V, bool FAST> struct mr_ld_axis {
     static constexpr bool kElemFast = FAST;
+    using PB = long long;
     const V *in;
     long long I;
     int n;
     __device__ int nsub() const { return 1; }
-    __device__ V operator()(long long p, int e, int) const
+    __device__ PB pencil(long long p, int) const
     {
         const long long o = p / I, i = p - o * I;
-        return in[i + I * (e + (long long)n * o)];
+        return i + I * (long long)n * o;
     }
+    __device__ V get(const PB &b, int e) const { return in[b + I * e]; }
 };
 
 // Zero-padded load: natural element e of a padded axis of length M reads
@@ -89,31 +99,37 @@ template <typename V, bool FAST> struct mr_ld_axis {
 // (the centering scatter of src/cuda/mr_kernels.cu:111-128), else 0.
 template <typename V, bool FAST> struct mr_ld_pad {
     static constexpr bool kElemFast = FAST;
+    using PB = long long;
     const V *in;
     long long I;
     int M, Ns, c;
     __device__ int nsub() const { return 1; }
-    __device__ V operator()(long long p, int e, int) const
+    __device__ PB pencil(long long p, int) const
+    {
+        const long long o = p / I, i = p - o * I;
+        return i + I * (long long)Ns * o;
+    }
+    __device__ V get(const PB &b, int e) const
     {
         int s = e + c;
         if (s >= M) s -= M;
-        if (s >= Ns) return cmake<V>(0, 0);
-        const long long o = p / I, i = p - o * I;
-        return in[i + I * (s + (long long)Ns * o)];
+        return (s < Ns) ? in[b + I * s] : cmake<V>(0, 0);
     }
 };
 
 // Full-length store to [I, n, O].
 template <typename V, bool FAST> struct mr_st_axis {
     static constexpr bool kElemFast = FAST;
+    using PB = long long;
     V *out;
     long long I;
     int n;
-    __device__ void put(long long p, int, int e, int, V v, V *) const
+    __device__ PB pencil(long long p, int) const
     {
         const long long o = p / I, i = p - o * I;
-        out[i + I * (e + (long long)n * o)] = v;
+        return i + I * (long long)n * o;
     }
+    __device__ void put(const PB &b, int, int e, int, V v, V *) const { out[b + I * e] = v; }
     __device__ void flush(long long, int, V *) const {}
 };
 
@@ -121,16 +137,20 @@ template <typename V, bool FAST> struct mr_st_axis {
 // when s < Ns; other elements are discarded (crop of mr_kernels.cu:158-165).
 template <typename V, bool FAST> struct mr_st_crop {
     static constexpr bool kElemFast = FAST;
+    using PB = long long;
     V *out;
     long long I;
     int M, Ns, c;
-    __device__ void put(long long p, int, int e, int, V v, V *) const
+    __device__ PB pencil(long long p, int) const
+    {
+        const long long o = p / I, i = p - o * I;
+        return i + I * (long long)Ns * o;
+    }
+    __device__ void put(const PB &b, int, int e, int, V v, V *) const
     {
         int s = e + c;
         if (s >= M) s -= M;
-        if (s >= Ns) return;
-        const long long o = p / I, i = p - o * I;
-        out[i + I * (s + (long long)Ns * o)] = v;
+        if (s < Ns) out[b + I * s] = v;
     }
     __device__ void flush(long long, int, V *) const {}
 };

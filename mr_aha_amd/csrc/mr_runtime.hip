@@ -55,6 +55,7 @@ template <typename R> int mr_fft_get(int n, mr_fft_len<R> *out)
         it = g_tw.emplace(key, d).first;
     }
     out->tw = reinterpret_cast<const V *>(it->second);
+    mr_fft_fill_divs(out);
     return MR_SUCCESS;
 }
 
