@@ -31,7 +31,12 @@ HBM_PEAK_GBS = 8000.0
 
 
 def stage_bytes(cfg, nc):
-    """Algorithmic HBM bytes of one launch of each stage for a group of nc coils."""
+    """Algorithmic HBM bytes of one launch of each profiled stage for a group
+    of nc coils: every array a stage must read or write, counted once.
+
+    Stack-of-stars (z transformed on the image side): F3 slot = Z1
+    (S x products, z-FFT), I3 slot = Z2 (inverse z-FFT, conj(S), coil sum).
+    """
     Nx, Ny, Nz = cfg.n_dim
     Mx, My, Mz = cfg.phi2_dim
     L = cfg.L
@@ -40,20 +45,26 @@ def stage_bytes(cfg, nc):
     N = Nx * Ny * Nz
     P = Mx * My * Mz
     c8 = 8
-    three = cfg.n_spatial == 3
+    img_v = N * V * c8                  # image-size volumes
     rows_x = Mx * Ny * Nz * V * c8      # x-transformed image rows
     rows_xy = Mx * My * Nz * V * c8     # x,y-transformed planes
     full = P * V * c8
-    b = {
-        "F1": N * L * c8 + N * nc * c8 + rows_x,
+    xs = N * L * c8 + N * nc * c8       # x and the maps, once
+    zfirst = cfg.traj_type == 2 and Mz == Nz
+    three = cfg.n_spatial == 3
+    if zfirst:
+        return {"F3": xs + img_v, "F1": img_v + rows_x, "F2": rows_x + rows_xy,
+                "BD": 2 * full + T * P * c8, "I2": rows_xy + rows_x, "I1": rows_x + img_v,
+                "I3": img_v + N * nc * c8 + N * L * c8}
+    return {
+        "F1": xs + rows_x,
         "F2": rows_x + rows_xy,
         "F3": (rows_xy + full) if three else 0,
         "BD": 2 * full + T * P * c8,
         "I3": (full + rows_xy) if three else 0,
         "I2": rows_xy + rows_x,
-        "I1": rows_x + N * nc * c8 + N * L * c8,
+        "I1": rows_x + xs,
     }
-    return b
 
 
 def apply_model_bytes(cfg):

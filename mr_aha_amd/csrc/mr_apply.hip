@@ -26,25 +26,28 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <utility>
 #include <vector>
 
 // ---------------------------------------------------------------------------
 // Fused pass functors
 // ---------------------------------------------------------------------------
 
-// F1 load: pencil p = yz + NyNz * v, v = l + L * c; natural element e.
+// F1 load: pencil p = v + V * yz (the V rank volumes of one image row are
+// neighbouring pencils, so the row of x_l and S_c is reused from cache),
+// v = l + L * c; natural element e.
 struct mr_ld_pad_sx {
     static constexpr bool kElemFast = true;
     struct PB {
         long long xo, so;
     };
     const float2 *x, *S;
-    long long N, NyNz;
+    long long N, NyNz, V;
     int Nx, Mx, cnx, L, coil0;
     __device__ int nsub() const { return 1; }
     __device__ PB pencil(long long p, int) const
     {
-        const long long v = p / NyNz, yz = p - v * NyNz;
+        const long long yz = p / V, v = p - yz * V;
         const int l = (int)(v % L), c = (int)(v / L);
         return PB{(long long)Nx * yz + N * l, (long long)Nx * yz + N * (coil0 + c)};
     }
@@ -56,7 +59,23 @@ struct mr_ld_pad_sx {
     }
 };
 
-// I1 load: pencil p = yz + NyNz * l, round `sub` = coil of the group.
+// Row store for F1's pencil order p = v + V * yz -> row yz + NyNz * v.
+struct mr_st_rows_vfast {
+    static constexpr bool kElemFast = true;
+    using PB = long long;
+    float2 *out;
+    long long NyNz, V;
+    int Mx;
+    __device__ PB pencil(long long p, int) const
+    {
+        const long long yz = p / V, v = p - yz * V;
+        return (long long)Mx * (yz + NyNz * v);
+    }
+    __device__ void put(const PB &b, int, int e, int, float2 v, float2 *) const { out[b + e] = v; }
+    __device__ void flush(long long, int, float2 *) const {}
+};
+
+// I1 load: pencil p = l + L * yz, round `sub` = coil of the group.
 struct mr_ld_row_coil {
     static constexpr bool kElemFast = true;
     using PB = long long;
@@ -66,7 +85,8 @@ struct mr_ld_row_coil {
     __device__ int nsub() const { return ncoil; }
     __device__ PB pencil(long long p, int sub) const
     {
-        return (long long)Mx * (p + NyNz * (long long)L * sub);
+        const long long yz = p / L, l = p - yz * L;
+        return (long long)Mx * (yz + NyNz * (l + (long long)L * sub));
     }
     __device__ float2 get(const PB &b, int e) const { return in[b + e]; }
 };
@@ -78,10 +98,10 @@ struct mr_st_crop_conj_sum {
     float2 *out;
     const float2 *S;
     long long N, NyNz;
-    int Nx, Mx, cnx, coil0, accumulate;
+    int Nx, Mx, cnx, coil0, accumulate, L;
     __device__ PB pencil(long long p, int sub) const
     {
-        const long long yz = p % NyNz;
+        const long long yz = p / L;
         return (long long)Nx * yz + N * (coil0 + sub);
     }
     __device__ void put(const PB &b, int pl, int e, int sub, float2 v, float2 *acc) const
@@ -97,11 +117,133 @@ struct mr_st_crop_conj_sum {
     {
         for (int pl = 0; pl < tp; ++pl) {
             const long long p = p0 + pl;
-            const long long l = p / NyNz, yz = p - l * NyNz;
+            const long long yz = p / L, l = p - yz * L;
             float2 *o = out + (long long)Nx * yz + N * l;
             const float2 *a = acc + (size_t)pl * Nx;
             for (int i = threadIdx.x; i < Nx; i += blockDim.x)
                 o[i] = accumulate ? cadd(o[i], a[i]) : a[i];
+        }
+    }
+};
+
+// Stack-of-stars: kz is Cartesian and never padded (Mz = Nz), so its
+// transform runs on the image side, before the x/y passes, and its inverse
+// after them: the z passes touch image-size data instead of the padded grid.
+// Pencils along z are enumerated p = lt + TXZ * (v + V * tile) with the image
+// location i = tile * TXZ + lt (TXZ a power of two dividing Nx*Ny): a tile of
+// TXZ neighbouring locations is revisited for every volume v by consecutive
+// workgroups, which reuse its rows of x and S from cache.
+struct mr_zmap {
+    long long V;
+    int lgTXZ;
+    __device__ void split(long long p, long long &i, long long &v) const
+    {
+        const long long lt = p & ((1LL << lgTXZ) - 1), rest = p >> lgTXZ;
+        const long long tile = rest / V;
+        v = rest - tile * V;
+        i = (tile << lgTXZ) + lt;
+    }
+};
+
+// Z1 load: S_c x_l along z in natural order (image z = (e + cnz) mod Nz).
+struct mr_ld_sx_z {
+    static constexpr bool kElemFast = false;
+    struct PB {
+        long long xo, so;
+    };
+    const float2 *x, *S;
+    mr_zmap zm;
+    long long NxNy, N;
+    int Nz, cnz, L, coil0;
+    __device__ int nsub() const { return 1; }
+    __device__ PB pencil(long long p, int) const
+    {
+        long long i, v;
+        zm.split(p, i, v);
+        const int l = (int)(v % L), c = (int)(v / L);
+        return PB{i + N * l, i + N * (coil0 + c)};
+    }
+    __device__ float2 get(const PB &b, int e) const
+    {
+        int z = e + cnz;
+        if (z >= Nz) z -= Nz;
+        const long long o = NxNy * z;
+        return cmul(S[b.so + o], x[b.xo + o]);
+    }
+};
+
+// Z1 store: [Nx*Ny, Nz (natural kz), V].
+struct mr_st_z {
+    static constexpr bool kElemFast = false;
+    using PB = long long;
+    float2 *out;
+    mr_zmap zm;
+    long long NxNy;
+    int Nz;
+    __device__ PB pencil(long long p, int) const
+    {
+        long long i, v;
+        zm.split(p, i, v);
+        return i + NxNy * Nz * v;
+    }
+    __device__ void put(const PB &b, int, int e, int, float2 val, float2 *) const
+    {
+        out[b + NxNy * e] = val;
+    }
+    __device__ void flush(long long, int, float2 *) const {}
+};
+
+// Z2 load: pencil (i, l), round `sub` = coil; [Nx*Ny, Nz, V] input.
+struct mr_ld_z_coil {
+    static constexpr bool kElemFast = false;
+    using PB = long long;
+    const float2 *in;
+    mr_zmap zm;  // V = L here
+    long long NxNy;
+    int Nz, L, ncoil;
+    __device__ int nsub() const { return ncoil; }
+    __device__ PB pencil(long long p, int sub) const
+    {
+        long long i, l;
+        zm.split(p, i, l);
+        return i + NxNy * Nz * (l + (long long)L * sub);
+    }
+    __device__ float2 get(const PB &b, int e) const { return in[b + NxNy * e]; }
+};
+
+// Z2 store: image z = (e + cnz) mod Nz, conj(S_c), coil sum in LDS; flush
+// writes AhAx[i, z, l].
+struct mr_st_z_conj_sum {
+    static constexpr bool kElemFast = false;
+    using PB = long long;
+    float2 *out;
+    const float2 *S;
+    mr_zmap zm;  // V = L
+    long long NxNy, N;
+    int Nz, cnz, coil0, accumulate;
+    __device__ PB pencil(long long p, int sub) const
+    {
+        long long i, l;
+        zm.split(p, i, l);
+        return i + N * (coil0 + sub);
+    }
+    __device__ void put(const PB &b, int pl, int e, int sub, float2 v, float2 *acc) const
+    {
+        int z = e + cnz;
+        if (z >= Nz) z -= Nz;
+        const float2 t = cmulc(S[b + NxNy * z], v);
+        float2 *a = acc + (size_t)pl * Nz + z;
+        *a = (sub == 0) ? t : cadd(*a, t);
+    }
+    __device__ void flush(long long p0, int tp, float2 *acc) const
+    {
+        for (int q = threadIdx.x; q < tp * Nz; q += blockDim.x) {
+            const int z = q / tp, pl = q - z * tp;
+            long long i, l;
+            zm.split(p0 + pl, i, l);
+            float2 *o = out + i + NxNy * z + N * l;
+            const float2 a = acc[(size_t)pl * Nz + z];
+            *o = accumulate ? cadd(*o, a) : a;
         }
     }
 };
@@ -539,15 +681,33 @@ static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, cons
         MR_TRY(mr_prof_mark(p, MR_ST_OTHER));
     }
 
+    const bool zfirst = cfg.traj_type == MR_TRAJ_STACK && Mz == Nz;
+    const long long NxNy = (long long)Nx * Ny;
+    int lgTXZ = 0;
+    while (lgTXZ < 6 && (NxNy % (2LL << lgTXZ)) == 0) ++lgTXZ;
+
     for (int coil0 = 0; coil0 < cfg.n_coils; coil0 += p->group) {
         const int nc = (cfg.n_coils - coil0 < p->group) ? cfg.n_coils - coil0 : p->group;
         const long long V = (long long)L * nc;
         float2 *A = p->bufA, *B = p->bufB;
 
-        // F1: pad * S, x-FFT -> A [Mx, Ny, Nz, V]
-        {
-            mr_ld_pad_sx ld{x, S, N, NyNz, Nx, Mx, d.c_n[0], L, coil0};
-            mr_st_axis<float2, true> st{A, 1, Mx};
+        if (zfirst) {
+            // Z1: S_c x_l, z-FFT on the image -> A [Nx, Ny, Nz(kz), V]
+            mr_zmap zm{V, lgTXZ};
+            mr_ld_sx_z ld{x, S, zm, NxNy, N, Nz, d.c_n[2], L, coil0};
+            mr_st_z st{A, zm, NxNy, Nz};
+            MR_TRY((mr_fft_pass<float, false>(p->fz, ld, st, NxNy * V, 0, 1 << lgTXZ, s, "Z1")));
+            MR_TRY(mr_prof_mark(p, MR_ST_F3));
+            // F1: zero-padded x-FFT -> B [Mx, Ny, Nz, V]
+            mr_ld_pad<float2, true> l1{A, 1, Mx, Nx, d.c_n[0]};
+            mr_st_axis<float2, true> s1{B, 1, Mx};
+            MR_TRY((mr_fft_pass<float, false>(p->fx, l1, s1, NyNz * V, 0, 8, s, "F1")));
+            MR_TRY(mr_prof_mark(p, MR_ST_F1));
+            std::swap(A, B);  // x-transformed rows now in A
+        } else {
+            // F1: pad * S, x-FFT -> A [Mx, Ny, Nz, V]
+            mr_ld_pad_sx ld{x, S, N, NyNz, V, Nx, Mx, d.c_n[0], L, coil0};
+            mr_st_rows_vfast st{A, NyNz, V, Mx};
             MR_TRY((mr_fft_pass<float, false>(p->fx, ld, st, NyNz * V, 0, 8, s, "F1")));
             MR_TRY(mr_prof_mark(p, MR_ST_F1));
         }
@@ -559,7 +719,7 @@ static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, cons
             MR_TRY(mr_prof_mark(p, MR_ST_F2));
         }
         float2 *K = B, *free_buf = A;
-        if (three) {
+        if (three && !zfirst) {
             // F3: z-FFT -> A [Mx, My, Mz, V]
             mr_ld_pad<float2, false> ld{B, (long long)Mx * My, Mz, Nz, d.c_n[2]};
             mr_st_axis<float2, false> st{A, (long long)Mx * My, Mz};
@@ -571,7 +731,7 @@ static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, cons
         MR_TRY(mr_blockdiag(p, K, nc, ker));
         MR_TRY(mr_prof_mark(p, MR_ST_BD));
         float2 *src = K;
-        if (three) {
+        if (three && !zfirst) {
             // I3: inverse z-FFT, keep Nz planes -> free_buf [Mx, My, Nz, V]
             mr_ld_axis<float2, false> ld{K, (long long)Mx * My, Mz};
             mr_st_crop<float2, false> st{free_buf, (long long)Mx * My, Mz, Nz, d.c_n[2]};
@@ -588,10 +748,22 @@ static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, cons
             MR_TRY((mr_fft_pass<float, true>(p->fy, ld, st, (long long)Mx * Nz * V, 0, 16, s, "I2")));
             MR_TRY(mr_prof_mark(p, MR_ST_I2));
         }
-        // I1: inverse x-FFT, crop, conj(S) and coil sum -> out
-        {
+        if (zfirst) {
+            // I1: inverse x-FFT, crop -> src [Nx, Ny, Nz, V]
+            mr_ld_axis<float2, true> l1{dst, 1, Mx};
+            mr_st_crop<float2, true> s1{src, 1, Mx, Nx, d.c_n[0]};
+            MR_TRY((mr_fft_pass<float, true>(p->fx, l1, s1, NyNz * V, 0, 8, s, "I1")));
+            MR_TRY(mr_prof_mark(p, MR_ST_I1));
+            // Z2: inverse z-FFT, conj(S_c), coil sum -> out
+            mr_zmap zm{L, lgTXZ};
+            mr_ld_z_coil ld{src, zm, NxNy, Nz, L, nc};
+            mr_st_z_conj_sum st{out, S, zm, NxNy, N, Nz, d.c_n[2], coil0, coil0 > 0 ? 1 : 0};
+            MR_TRY((mr_fft_pass<float, true>(p->fz, ld, st, NxNy * L, (size_t)Nz, 1 << lgTXZ, s, "Z2")));
+            MR_TRY(mr_prof_mark(p, MR_ST_I3));
+        } else {
+            // I1: inverse x-FFT, crop, conj(S) and coil sum -> out
             mr_ld_row_coil ld{dst, NyNz, Mx, L, nc};
-            mr_st_crop_conj_sum st{out, S, N, NyNz, Nx, Mx, d.c_n[0], coil0, coil0 > 0 ? 1 : 0};
+            mr_st_crop_conj_sum st{out, S, N, NyNz, Nx, Mx, d.c_n[0], coil0, coil0 > 0 ? 1 : 0, L};
             MR_TRY((mr_fft_pass<float, true>(p->fx, ld, st, NyNz * L, (size_t)Nx, 8, s, "I1")));
             MR_TRY(mr_prof_mark(p, MR_ST_I1));
         }

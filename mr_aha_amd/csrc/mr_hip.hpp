@@ -31,7 +31,7 @@ void mr_pop_device(int prev);
 int mr_mem_budget(const mr_config *cfg, size_t *budget);
 
 // Launch one axis pass of the FFT engine on `stream`: the register-staged
-// single-buffer kernel for smooth FP32 lengths when MR_AHA_HIP_FFT_FAST=1,
+// single-buffer kernel for smooth FP32 lengths (MR_AHA_HIP_FFT_FAST=0 disables it),
 // the ping-pong kernel otherwise (any length, FP64). TX is a power of two.
 template <typename R, bool INV, class Load, class Store>
 int mr_fft_pass(const mr_fft_len<R> &f, const Load &ld, const Store &st, long long npencil,
@@ -40,7 +40,7 @@ int mr_fft_pass(const mr_fft_len<R> &f, const Load &ld, const Store &st, long lo
     using V = typename cvec<R>::T;
     if (npencil <= 0) return MR_SUCCESS;
     if (want_tx > MR_FFT_MAXTX) want_tx = MR_FFT_MAXTX;
-    static const bool fast_on = mr_env_ll("MR_AHA_HIP_FFT_FAST", 0) != 0;
+    static const bool fast_on = mr_env_ll("MR_AHA_HIP_FFT_FAST", 1) != 0;
     bool smooth = sizeof(R) == 4 && fast_on;
     for (int i = 0; i < f.nst && smooth; ++i) smooth = (f.radix[i] <= 8 && f.radix[i] != 6);
     int TX;
