@@ -314,7 +314,7 @@ __device__ __forceinline__ void mr_bd_walk(const float2 (&v)[NC][L], float2 (&y)
 // handles location p for coils slot, slot + 8, ...
 #define MR_BD_TP 32
 template <int L, bool MASKED>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(512)
 mr_k_bd(float2 *__restrict__ K, const float2 *__restrict__ ker, const int *__restrict__ inv,
         long long kstride, mr_dims d, int ncoil)
 {
@@ -635,9 +635,11 @@ static int mr_blockdiag(mr_aha_plan p, float2 *K, int nc, const float2 *ker)
     const long long kstride = masked ? (long long)p->cfg.num_masked : d.pad_size;
     mr_bd_fn fn = masked ? mr_bd_pick<true>(d.L) : mr_bd_pick<false>(d.L);
     if (fn) {
+        // one thread slot per coil (up to 16), so no slot idles in a second round
+        const int slots = (nc < 16) ? nc : 16;
         const long long blocks = mr_ceil_div(d.pad_size, MR_BD_TP);
-        hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(256), 0, p->stream, K, ker, p->d_inv,
-                           kstride, d, nc);
+        hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(MR_BD_TP * slots), 0, p->stream, K,
+                           ker, p->d_inv, kstride, d, nc);
         MR_LAUNCH_CHECK("mr_k_bd");
         return MR_SUCCESS;
     }

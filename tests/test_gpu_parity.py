@@ -250,3 +250,29 @@ def test_no_leak(mr):
     torch.cuda.synchronize()
     after = torch.cuda.mem_get_info()[0]
     assert before - after <= 8 << 20
+
+
+@pytest.mark.parametrize("name", ["2d_50x40_L3_c3", "2d_33x27_L1_c2_odd", "3d_25x21x15_L3_c2_odd",
+                                  "stack_31x36x9_L4_c2_odd", "checkop_stack"])
+def test_nufft_method(mr, golden, name):
+    """method='nufft' against the exact NUDFT normal operator (the reference's
+    library-vs-Python NUFFT tolerance 1e-5) and against the Toeplitz method
+    (1e-4, tests/check_op.py:92-94)."""
+    print(f"[nufft] {name}", flush=True)
+    gd = golden(name)
+    g, traj, phi2, x, S = case(name)
+    cfg = cfg_for(mr, g)
+    dx, dS, dphi2, dtraj = dev(x), dev(S), dev(phi2), dev(traj)
+    for share in (True, False):
+        with mr.create_op(cfg, dS, dphi2, dtraj, method="nufft", share_primary=share) as op:
+            y = host(op(dx))
+            assert rel(y, gd["AhAx_nudft"]) < 1e-5
+            inf = op.info
+            assert inf["toep_source"] == "none" and inf["method"] == "nufft" and inf["group"] == [1]
+            with pytest.raises(mr.MRError, match="no kernel"):
+                op.kernel(0)
+    K = mr.calc_toep(cfg, dphi2, dtraj)
+    assert rel(y, host(mr.aha_toeplitz(cfg, dx, dS, K))) < 1e-4
+    with mr.create_op(cfg, dS, dphi2, dtraj, method="nufft",
+                      nufft_opts={"upsampfac": 2.0, "gpu_method": 1}) as op:
+        assert rel(host(op(dx)), gd["AhAx_nudft"]) < 1e-5
