@@ -306,39 +306,38 @@ __device__ __forceinline__ void mr_bd_walk(const float2 (&v)[NC][L], float2 (&y)
 // centered. Masked: inv[loc] is the kernel row of natural location loc or -1
 // (a location outside the mask multiplies by zero).
 //
-// One workgroup per tile of MR_BD_TP consecutive locations: the tile's T
-// packed kernel entries are staged once in LDS (coalesced: consecutive
-// natural locations are consecutive centered locations except at the one
-// wrap point of the x axis) and every coil of the group is multiplied from
-// LDS, so the kernel streams from HBM once per apply. Thread (p, slot)
-// handles location p for coils slot, slot + 8, ...
-#define MR_BD_TP 32
-template <int L, bool MASKED>
-__global__ void __launch_bounds__(512)
+// One workgroup per tile of TP consecutive locations: the tile's T packed
+// kernel entries are staged once in LDS (coalesced: consecutive natural
+// locations are consecutive centered locations except at the one wrap point
+// of the x axis) and every coil of the group is multiplied from LDS, so the
+// kernel streams from HBM once per apply. Thread (p, slot) handles location
+// p for coils slot, slot + nslot, ...
+template <int L, int TP, bool MASKED>
+__global__ void __launch_bounds__(256)
 mr_k_bd(float2 *__restrict__ K, const float2 *__restrict__ ker, const int *__restrict__ inv,
         long long kstride, mr_dims d, int ncoil)
 {
     constexpr int T = L * (L + 1) / 2;
-    __shared__ float2 kt[T * MR_BD_TP];
-    __shared__ long long kb[MR_BD_TP];
+    __shared__ float2 kt[T * TP];
+    __shared__ long long kb[TP];
     const long long P = d.pad_size;
-    const long long loc0 = mr_xcd_block(blockIdx.x, gridDim.x) * MR_BD_TP;
-    if (threadIdx.x < MR_BD_TP) {
+    const long long loc0 = mr_xcd_block(blockIdx.x, gridDim.x) * TP;
+    if (threadIdx.x < TP) {
         const long long loc = loc0 + threadIdx.x;
         long long b = -1;
         if (loc < P) b = MASKED ? (long long)inv[loc] : mr_nat_to_cent_lin(loc, d);
         kb[threadIdx.x] = b;
     }
     __syncthreads();
-    for (int f = threadIdx.x; f < T * MR_BD_TP; f += blockDim.x) {
-        const int t = f / MR_BD_TP, p = f - t * MR_BD_TP;
+    for (int f = threadIdx.x; f < T * TP; f += blockDim.x) {
+        const int t = f / TP, p = f - t * TP;
         const long long b = kb[p];
         kt[f] = (b >= 0) ? ker[b + kstride * t] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    const int p = threadIdx.x % MR_BD_TP;
-    const int slot = threadIdx.x / MR_BD_TP;
-    const int nslot = blockDim.x / MR_BD_TP;
+    const int p = threadIdx.x % TP;
+    const int slot = threadIdx.x / TP;
+    const int nslot = blockDim.x / TP;
     const long long loc = loc0 + p;
     if (loc >= P) return;
     for (int c = slot; c < ncoil; c += nslot) {
@@ -346,7 +345,7 @@ mr_k_bd(float2 *__restrict__ K, const float2 *__restrict__ ker, const int *__res
         float2 v[1][L], y[1][L];
 #pragma unroll
         for (int l = 0; l < L; ++l) v[0][l] = g[P * l];
-        mr_bd_walk<L, 1>(v, y, kt + p, MR_BD_TP, d.tri_upper);
+        mr_bd_walk<L, 1>(v, y, kt + p, TP, d.tri_upper);
 #pragma unroll
         for (int l = 0; l < L; ++l) g[P * l] = y[0][l];
     }
@@ -422,10 +421,10 @@ __global__ void mr_k_mask_inverse(int *inv, const int64_t *mask, long long nm, m
 
 typedef void (*mr_bd_fn)(float2 *, const float2 *, const int *, long long, mr_dims, int);
 
-template <bool MASKED> static mr_bd_fn mr_bd_pick(int L)
+template <int TP, bool MASKED> static mr_bd_fn mr_bd_pick_tp(int L)
 {
     switch (L) {
-#define MR_BD_CASE(n) case n: return mr_k_bd<n, MASKED>;
+#define MR_BD_CASE(n) case n: return mr_k_bd<n, TP, MASKED>;
         MR_BD_CASE(1) MR_BD_CASE(2) MR_BD_CASE(3) MR_BD_CASE(4)
         MR_BD_CASE(5) MR_BD_CASE(6) MR_BD_CASE(7) MR_BD_CASE(8)
         MR_BD_CASE(9) MR_BD_CASE(10) MR_BD_CASE(11) MR_BD_CASE(12)
@@ -433,6 +432,13 @@ template <bool MASKED> static mr_bd_fn mr_bd_pick(int L)
 #undef MR_BD_CASE
     default: return nullptr;
     }
+}
+
+template <bool MASKED> static mr_bd_fn mr_bd_pick(int L, int TP)
+{
+    if (TP == 16) return mr_bd_pick_tp<16, MASKED>(L);
+    if (TP == 64) return mr_bd_pick_tp<64, MASKED>(L);
+    return mr_bd_pick_tp<32, MASKED>(L);
 }
 
 // ---------------------------------------------------------------------------
@@ -633,13 +639,19 @@ static int mr_blockdiag(mr_aha_plan p, float2 *K, int nc, const float2 *ker)
     const mr_dims d = p->dims;
     const bool masked = p->cfg.is_masked != 0;
     const long long kstride = masked ? (long long)p->cfg.num_masked : d.pad_size;
-    mr_bd_fn fn = masked ? mr_bd_pick<true>(d.L) : mr_bd_pick<false>(d.L);
+    // Tile width (locations) and coil slots per workgroup; MR_AHA_HIP_BD_TP
+    // (16, 32, 64) and MR_AHA_HIP_BD_SLOTS override them (read per call).
+    int TP = (int)mr_env_ll("MR_AHA_HIP_BD_TP", 32);
+    if (TP != 16 && TP != 64) TP = 32;
+    int slots = (int)mr_env_ll("MR_AHA_HIP_BD_SLOTS", 8);
+    if (slots < 1) slots = 1;
+    if (slots > nc) slots = nc;
+    while (slots > 1 && TP * slots > 256) --slots;
+    mr_bd_fn fn = masked ? mr_bd_pick<true>(d.L, TP) : mr_bd_pick<false>(d.L, TP);
     if (fn) {
-        // one thread slot per coil (up to 16), so no slot idles in a second round
-        const int slots = (nc < 16) ? nc : 16;
-        const long long blocks = mr_ceil_div(d.pad_size, MR_BD_TP);
-        hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(MR_BD_TP * slots), 0, p->stream, K,
-                           ker, p->d_inv, kstride, d, nc);
+        const long long blocks = mr_ceil_div(d.pad_size, TP);
+        hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(TP * slots), 0, p->stream, K, ker,
+                           p->d_inv, kstride, d, nc);
         MR_LAUNCH_CHECK("mr_k_bd");
         return MR_SUCCESS;
     }
