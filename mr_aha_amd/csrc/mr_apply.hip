@@ -641,9 +641,14 @@ static int mr_blockdiag(mr_aha_plan p, float2 *K, int nc, const float2 *ker)
     const long long kstride = masked ? (long long)p->cfg.num_masked : d.pad_size;
     // Tile width (locations) and coil slots per workgroup; MR_AHA_HIP_BD_TP
     // (16, 32, 64) and MR_AHA_HIP_BD_SLOTS override them (read per call).
+    // Default slots: the fewest that still give every slot the same number of
+    // coils, ceil(nc / ceil(nc / smax)) with smax = 256 / TP, so no slot idles
+    // through a last round (12 coils at TP 32: 6 slots of 2 coils each).
     int TP = (int)mr_env_ll("MR_AHA_HIP_BD_TP", 32);
     if (TP != 16 && TP != 64) TP = 32;
-    int slots = (int)mr_env_ll("MR_AHA_HIP_BD_SLOTS", 8);
+    const int smax = 256 / TP;
+    int slots = (int)mr_ceil_div(nc, mr_ceil_div(nc, smax));
+    slots = (int)mr_env_ll("MR_AHA_HIP_BD_SLOTS", slots);
     if (slots < 1) slots = 1;
     if (slots > nc) slots = nc;
     while (slots > 1 && TP * slots > 256) --slots;
