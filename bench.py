@@ -30,11 +30,16 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 
-def stage_bytes(cfg, nc):
+def stage_bytes(cfg, nc, xbd):
     """Algorithmic HBM bytes of one launch of each profiled stage for a group
-    of nc coils: every array a stage must read or write, counted once.
+    of nc coils: every array a stage must read or write, counted once (the
+    kernel once per launch, however many coil groups of a row share it).
 
-    Stack-of-stars (z transformed on the image side): F3 slot = Z1
+    xbd > 0 (fused x pass): F3 = outer-axis FFT (S x products, zero-pad),
+    F2 = y-FFT, BD = fused x-FFT + block multiply + inverse x-FFT in place,
+    I2 = inverse y-FFT, I3 = inverse outer FFT with conj(S) and the coil sum;
+    in 2D the outer axis is y (F2 and I2 only).
+    xbd == 0, stack-of-stars (z transformed on the image side): F3 slot = Z1
     (S x products, z-FFT), I3 slot = Z2 (inverse z-FFT, conj(S), coil sum).
     """
     Nx, Ny, Nz = cfg.n_dim
@@ -45,22 +50,31 @@ def stage_bytes(cfg, nc):
     N = Nx * Ny * Nz
     P = Mx * My * Mz
     c8 = 8
+    kern = T * (cfg.num_masked if cfg.is_masked else P) * c8
+    xs = N * L * c8 + N * nc * c8       # x and the maps, once
+    outs = N * nc * c8 + N * L * c8     # maps and AhAx
+    three = cfg.n_spatial == 3
+    if xbd:
+        if not three:
+            b = Nx * My * V * c8
+            return {"F2": xs + b, "BD": 2 * b + kern, "I2": b + outs}
+        a = Nx * Ny * Mz * V * c8
+        b = Nx * My * Mz * V * c8
+        return {"F3": xs + a, "F2": a + b, "BD": 2 * b + kern, "I2": b + a, "I3": a + outs}
     img_v = N * V * c8                  # image-size volumes
     rows_x = Mx * Ny * Nz * V * c8      # x-transformed image rows
     rows_xy = Mx * My * Nz * V * c8     # x,y-transformed planes
     full = P * V * c8
-    xs = N * L * c8 + N * nc * c8       # x and the maps, once
     zfirst = cfg.traj_type == 2 and Mz == Nz
-    three = cfg.n_spatial == 3
     if zfirst:
         return {"F3": xs + img_v, "F1": img_v + rows_x, "F2": rows_x + rows_xy,
-                "BD": 2 * full + T * P * c8, "I2": rows_xy + rows_x, "I1": rows_x + img_v,
-                "I3": img_v + N * nc * c8 + N * L * c8}
+                "BD": 2 * full + kern, "I2": rows_xy + rows_x, "I1": rows_x + img_v,
+                "I3": img_v + outs}
     return {
         "F1": xs + rows_x,
         "F2": rows_x + rows_xy,
         "F3": (rows_xy + full) if three else 0,
-        "BD": 2 * full + T * P * c8,
+        "BD": 2 * full + kern,
         "I3": (full + rows_xy) if three else 0,
         "I2": rows_xy + rows_x,
         "I1": rows_x + xs,
@@ -170,7 +184,8 @@ def main():
     value = 1000.0 / ms_per_step
 
     group = info["group"][0]
-    sb = stage_bytes(cfg, group)
+    xbd = int(_lib.lib.mr_op_xbd_group(op._handle))
+    sb = stage_bytes(cfg, group, xbd)
     stages = {}
     for i, name in enumerate(_lib.PROFILE_STAGES):
         if calls[i]:
@@ -197,6 +212,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64 (fp32 complex)", "data": "synthetic",
         "config": {"workload": workload, "precision": "fp32 kernel", "coil_group": group,
+                   "pipeline": (f"fused x pass, {xbd} coil(s) per workgroup" if xbd
+                                else "separate FFT and block passes"),
                    "kernel_build_s": round(info["toep_ms"] / 1e3, 2)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -24,6 +24,17 @@ MR_EXPORT int mr_aha_plan_profile(mr_aha_plan plan, double *ms, int *calls, int 
 /* Same for the plan inside a Toeplitz operator. */
 MR_EXPORT int mr_op_profile(mr_op op, double *ms, int *calls, int n);
 
+/* Apply pipeline chosen by a plan: the number of coils per workgroup of the
+ * fused x pass (forward x-FFT, block multiply and inverse x-FFT of one
+ * k-space row in one kernel), or 0 when the plan runs separate FFT and
+ * block passes. With the fused pass the stages timed as F3/F2/BD/I2/I3 are
+ * the outer-axis FFT, the y-FFT, the fused x pass, the inverse y-FFT and the
+ * inverse outer-axis FFT (2D: F2 = y-FFT with the sensitivity product,
+ * I2 = inverse y-FFT with the coil sum). Negative on error. No counterpart
+ * in the reference. */
+MR_EXPORT int mr_aha_plan_xbd_group(mr_aha_plan plan);
+MR_EXPORT int mr_op_xbd_group(mr_op op);
+
 #ifdef __cplusplus
 }
 #endif

@@ -126,11 +126,14 @@ struct mr_st_crop_conj_sum {
     }
 };
 
-// Stack-of-stars: kz is Cartesian and never padded (Mz = Nz), so its
-// transform runs on the image side, before the x/y passes, and its inverse
-// after them: the z passes touch image-size data instead of the padded grid.
-// Pencils along z are enumerated p = lt + TXZ * (v + V * tile) with the image
-// location i = tile * TXZ + lt (TXZ a power of two dividing Nx*Ny): a tile of
+// Outer-axis passes. The slowest spatial axis (z in 3D and stack-of-stars,
+// y in 2D) is transformed on the image side, before the inner axes, with the
+// sensitivity product fused into its load, and its inverse runs last with
+// crop, conj(S) and the coil sum fused into its store. "plane" is the stride
+// of that axis (Nx*Ny, or Nx in 2D); Nz its image length and Mz its transform
+// length (Mz = Nz for stack-of-stars kz, which is never padded).
+// Pencils are enumerated p = lt + TXZ * (v + V * tile) with the in-plane
+// location i = tile * TXZ + lt (TXZ a power of two dividing plane): a tile of
 // TXZ neighbouring locations is revisited for every volume v by consecutive
 // workgroups, which reuse its rows of x and S from cache.
 struct mr_zmap {
@@ -145,7 +148,8 @@ struct mr_zmap {
     }
 };
 
-// Z1 load: S_c x_l along z in natural order (image z = (e + cnz) mod Nz).
+// Outer forward load: S_c x_l along the axis in natural order, zero-padded
+// (image z = (e + cnz) mod Mz when < Nz).
 struct mr_ld_sx_z {
     static constexpr bool kElemFast = false;
     struct PB {
@@ -154,7 +158,7 @@ struct mr_ld_sx_z {
     const float2 *x, *S;
     mr_zmap zm;
     long long NxNy, N;
-    int Nz, cnz, L, coil0;
+    int Nz, Mz, cnz, L, coil0;
     __device__ int nsub() const { return 1; }
     __device__ PB pencil(long long p, int) const
     {
@@ -166,25 +170,26 @@ struct mr_ld_sx_z {
     __device__ float2 get(const PB &b, int e) const
     {
         int z = e + cnz;
-        if (z >= Nz) z -= Nz;
+        if (z >= Mz) z -= Mz;
+        if (z >= Nz) return make_float2(0.f, 0.f);
         const long long o = NxNy * z;
         return cmul(S[b.so + o], x[b.xo + o]);
     }
 };
 
-// Z1 store: [Nx*Ny, Nz (natural kz), V].
+// Outer forward store: [plane, Mz (natural k), V].
 struct mr_st_z {
     static constexpr bool kElemFast = false;
     using PB = long long;
     float2 *out;
     mr_zmap zm;
     long long NxNy;
-    int Nz;
+    int Mz;
     __device__ PB pencil(long long p, int) const
     {
         long long i, v;
         zm.split(p, i, v);
-        return i + NxNy * Nz * v;
+        return i + NxNy * Mz * v;
     }
     __device__ void put(const PB &b, int, int e, int, float2 val, float2 *) const
     {
@@ -193,26 +198,26 @@ struct mr_st_z {
     __device__ void flush(long long, int, float2 *) const {}
 };
 
-// Z2 load: pencil (i, l), round `sub` = coil; [Nx*Ny, Nz, V] input.
+// Outer inverse load: pencil (i, l), round `sub` = coil; [plane, Mz, V] input.
 struct mr_ld_z_coil {
     static constexpr bool kElemFast = false;
     using PB = long long;
     const float2 *in;
     mr_zmap zm;  // V = L here
     long long NxNy;
-    int Nz, L, ncoil;
+    int Mz, L, ncoil;
     __device__ int nsub() const { return ncoil; }
     __device__ PB pencil(long long p, int sub) const
     {
         long long i, l;
         zm.split(p, i, l);
-        return i + NxNy * Nz * (l + (long long)L * sub);
+        return i + NxNy * Mz * (l + (long long)L * sub);
     }
     __device__ float2 get(const PB &b, int e) const { return in[b + NxNy * e]; }
 };
 
-// Z2 store: image z = (e + cnz) mod Nz, conj(S_c), coil sum in LDS; flush
-// writes AhAx[i, z, l].
+// Outer inverse store: image z = (e + cnz) mod Mz (elements with z >= Nz
+// are cropped), conj(S_c), coil sum in LDS; flush writes AhAx[i, z, l].
 struct mr_st_z_conj_sum {
     static constexpr bool kElemFast = false;
     using PB = long long;
@@ -220,7 +225,7 @@ struct mr_st_z_conj_sum {
     const float2 *S;
     mr_zmap zm;  // V = L
     long long NxNy, N;
-    int Nz, cnz, coil0, accumulate;
+    int Nz, Mz, cnz, coil0, accumulate;
     __device__ PB pencil(long long p, int sub) const
     {
         long long i, l;
@@ -230,7 +235,8 @@ struct mr_st_z_conj_sum {
     __device__ void put(const PB &b, int pl, int e, int sub, float2 v, float2 *acc) const
     {
         int z = e + cnz;
-        if (z >= Nz) z -= Nz;
+        if (z >= Mz) z -= Mz;
+        if (z >= Nz) return;
         const float2 t = cmulc(S[b + NxNy * z], v);
         float2 *a = acc + (size_t)pl * Nz + z;
         *a = (sub == 0) ? t : cadd(*a, t);
@@ -442,6 +448,133 @@ template <bool MASKED> static mr_bd_fn mr_bd_pick(int L, int TP)
 }
 
 // ---------------------------------------------------------------------------
+// Fused x pass (x-last pipeline)
+// ---------------------------------------------------------------------------
+
+// When the outer axes are transformed on the image side first, the x axis is
+// the last one: for one k-space row r = (ky, kz) (natural order) the forward
+// x-FFT, the block multiply of the row's Mx locations and the inverse x-FFT
+// need nothing outside the row. One workgroup takes the row for a group of up
+// to G coils of Y [Nx, R, V] (R = My*Mz rows, V = L * coils):
+//   load   the L*G pencils of Nx image columns, zero-padded to Mx in natural
+//          order (pad map of src/cuda/mr_kernels.cu:111-128)
+//   FFT    forward, radix stages in LDS
+//   block  v <- v B(k) at each of the Mx locations; the row's T*Mx kernel
+//          entries are read coalesced (centered kx runs contiguously except at
+//          the one wrap point) and shared, through L2, by the G coils and by
+//          the coil groups of the row, which are consecutive workgroups of
+//          one XCD (src/cuda/mr_kernels_bd.cu:74-139 for the walk)
+//   IFFT   inverse
+//   store  the Nx image columns back in place (crop of mr_kernels.cu:158-165)
+// Padded k-space never reaches HBM: per row and coil the pass moves 2*L*Nx
+// values plus the kernel row. Twiddles are read from the global table (L1
+// resident) so LDS holds exactly G*L*Mx values and two 80 KB workgroups can
+// share a CU.
+#define MR_XBD_THREADS 512
+#define MR_XBD_EMAX 16384
+
+template <bool INV>
+__device__ __forceinline__ void mr_xbd_fft(float2 *A, const mr_fft_len<float> &f, int tp)
+{
+    int Ns = 1;
+    for (int s = 0; s < f.nst; ++s) {
+        const int RR = f.radix[s];
+        const mr_fdiv &md = f.mdiv[s], &nd = f.nsdiv[s];
+        switch (RR) {
+        case 2: mr_stage_reg<2, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tw, f.n, f.n, tp, Ns, md, nd); break;
+        case 3: mr_stage_reg<3, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tw, f.n, f.n, tp, Ns, md, nd); break;
+        case 4: mr_stage_reg<4, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tw, f.n, f.n, tp, Ns, md, nd); break;
+        case 5: mr_stage_reg<5, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tw, f.n, f.n, tp, Ns, md, nd); break;
+        case 7: mr_stage_reg<7, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tw, f.n, f.n, tp, Ns, md, nd); break;
+        default: mr_stage_reg<8, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tw, f.n, f.n, tp, Ns, md, nd); break;
+        }
+        Ns *= RR;
+    }
+}
+
+template <int L, bool MASKED>
+__global__ void __launch_bounds__(MR_XBD_THREADS)
+mr_k_xbd(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__restrict__ inv,
+         long long kstride, mr_dims d, mr_fft_len<float> fx, mr_fdiv nxdiv, long long R, int G,
+         int ncoil, int ncg)
+{
+    extern __shared__ __align__(16) unsigned char mr_smem[];
+    float2 *A = reinterpret_cast<float2 *>(mr_smem);
+    const int Nx = d.n[0], Mx = d.m[0], cnx = d.c_n[0];
+    const long long q = mr_xcd_block(blockIdx.x, gridDim.x);
+    const long long r = q / ncg;
+    const int c0 = (int)(q - r * ncg) * G;
+    const int gn = (ncoil - c0 < G) ? ncoil - c0 : G;
+    const int tp = gn * L;
+    // pencil pl = g*L + l of the workgroup is volume v = L*c0 + pl
+    float2 *Yr = Y + (long long)Nx * r + (long long)Nx * R * L * c0;
+    const long long pstride = (long long)Nx * R;
+
+    for (int i = threadIdx.x; i < tp * Mx; i += MR_XBD_THREADS) {
+        const int pl = (int)mr_fdiv_q(fx.ndiv, (unsigned)i), e = i - pl * Mx;
+        int s = e + cnx;
+        if (s >= Mx) s -= Mx;
+        A[i] = (s < Nx) ? Yr[pstride * pl + s] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    mr_xbd_fft<false>(A, fx, tp);
+
+    const int My = d.m[1];
+    const int jy = (int)(r % My), jz = (int)(r / My);
+    int ky = jy + d.c_m[1]; if (ky >= My) ky -= My;
+    int kz = jz + d.c_m[2]; if (kz >= d.m[2]) kz -= d.m[2];
+    const long long rowc = (long long)Mx * (ky + (long long)My * kz);
+    for (int w = threadIdx.x; w < gn * Mx; w += MR_XBD_THREADS) {
+        const int g = (int)mr_fdiv_q(fx.ndiv, (unsigned)w), e = w - g * Mx;
+        long long b;
+        if (MASKED) {
+            b = inv[e + (long long)Mx * r];
+        } else {
+            int kx = e + d.c_m[0];
+            if (kx >= Mx) kx -= Mx;
+            b = rowc + kx;
+        }
+        float2 *a = A + (size_t)g * L * Mx + e;
+        float2 v[1][L], y[1][L];
+#pragma unroll
+        for (int l = 0; l < L; ++l) v[0][l] = a[l * Mx];
+        if (b >= 0) {
+            mr_bd_walk<L, 1>(v, y, ker + b, kstride, d.tri_upper);
+        } else {
+#pragma unroll
+            for (int l = 0; l < L; ++l) y[0][l] = make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int l = 0; l < L; ++l) a[l * Mx] = y[0][l];
+    }
+    __syncthreads();
+    mr_xbd_fft<true>(A, fx, tp);
+
+    for (int i = threadIdx.x; i < tp * Nx; i += MR_XBD_THREADS) {
+        const int pl = (int)mr_fdiv_q(nxdiv, (unsigned)i), s = i - pl * Nx;
+        int e = s - cnx;
+        if (e < 0) e += Mx;
+        Yr[pstride * pl + s] = A[pl * Mx + e];
+    }
+}
+
+typedef void (*mr_xbd_fn)(float2 *, const float2 *, const int *, long long, mr_dims,
+                          mr_fft_len<float>, mr_fdiv, long long, int, int, int);
+
+template <bool MASKED> static mr_xbd_fn mr_xbd_pick(int L)
+{
+    switch (L) {
+#define MR_XBD_CASE(n) case n: return mr_k_xbd<n, MASKED>;
+        MR_XBD_CASE(1) MR_XBD_CASE(2) MR_XBD_CASE(3) MR_XBD_CASE(4)
+        MR_XBD_CASE(5) MR_XBD_CASE(6) MR_XBD_CASE(7) MR_XBD_CASE(8)
+        MR_XBD_CASE(9) MR_XBD_CASE(10) MR_XBD_CASE(11) MR_XBD_CASE(12)
+        MR_XBD_CASE(13) MR_XBD_CASE(14) MR_XBD_CASE(15) MR_XBD_CASE(16)
+#undef MR_XBD_CASE
+    default: return nullptr;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Plan
 // ---------------------------------------------------------------------------
 
@@ -469,6 +602,9 @@ struct mr_aha_plan_s {
     mr_fft_len<float> fx, fy, fz;
     mr_prof prof;
     bool prof_on;
+    int xbd_g;          // coils per workgroup of the fused x pass; 0 = separate passes
+    size_t xbd_lds;     // its dynamic LDS bytes
+    mr_fdiv nxdiv;
 };
 
 static int mr_prof_mark(mr_aha_plan p, int stage)
@@ -561,6 +697,49 @@ static void mr_plan_free_buffers(mr_aha_plan p)
     p->d_inv = nullptr;
 }
 
+// Decide whether this plan runs the fused x pass: rank up to 16, an x length
+// made of the unrolled radices, and one coil's padded row within the LDS of a
+// workgroup. Default G: the most coils that keep a workgroup within 80 KB of
+// LDS, so two workgroups share a CU. MR_AHA_HIP_XBD=0 selects the separate
+// passes; MR_AHA_HIP_XBD_G sets G.
+static int mr_xbd_setup(mr_aha_plan p)
+{
+    p->xbd_g = 0;
+    p->xbd_lds = 0;
+    const mr_config &cfg = p->cfg;
+    const int L = cfg.L, Mx = p->dims.m[0];
+    if (mr_env_ll("MR_AHA_HIP_XBD", 1) == 0 || L > 16) return MR_SUCCESS;
+    for (int i = 0; i < p->fx.nst; ++i)
+        if (p->fx.radix[i] > 8 || p->fx.radix[i] == 6) return MR_SUCCESS;
+    const long long per_elems = (long long)L * Mx;
+    const size_t per = (size_t)per_elems * sizeof(float2);
+    if (per_elems > MR_XBD_EMAX) return MR_SUCCESS;
+    int dev = 0, maxlds = 0;
+    MR_HIP_CHECK(hipGetDevice(&dev));
+    MR_HIP_CHECK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (per > (size_t)maxlds) return MR_SUCCESS;
+    long long gmax = MR_XBD_EMAX / per_elems;
+    if (gmax > (long long)(maxlds / per)) gmax = (long long)(maxlds / per);
+    if (gmax > cfg.n_coils) gmax = cfg.n_coils;
+    long long g = (long long)(81920 / per);
+    g = mr_env_ll("MR_AHA_HIP_XBD_G", g);
+    if (g > gmax) g = gmax;
+    if (g < 1) g = 1;
+    const size_t lds = per * (size_t)g;
+    mr_xbd_fn fn = cfg.is_masked ? mr_xbd_pick<true>(L) : mr_xbd_pick<false>(L);
+    if (!fn) return MR_SUCCESS;
+    if (lds > 65536 &&
+        hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess) {
+        (void)hipGetLastError();
+        return MR_SUCCESS;
+    }
+    p->xbd_g = (int)g;
+    p->xbd_lds = lds;
+    p->nxdiv = mr_fdiv_make((unsigned)p->dims.n[0]);
+    return MR_SUCCESS;
+}
+
 static int mr_plan_create_impl(const mr_config *cfg, mr_aha_plan *out)
 {
     MR_TRY(mr_config_validate(cfg));
@@ -572,22 +751,36 @@ static int mr_plan_create_impl(const mr_config *cfg, mr_aha_plan *out)
     int rc = mr_fft_get<float>(cfg->phi2_dim[0], &p->fx);
     if (rc == MR_SUCCESS) rc = mr_fft_get<float>(cfg->phi2_dim[1], &p->fy);
     if (rc == MR_SUCCESS) rc = mr_fft_get<float>(cfg->phi2_dim[2], &p->fz);
+    if (rc == MR_SUCCESS) rc = mr_xbd_setup(p);
     if (rc != MR_SUCCESS) { delete p; return rc; }
 
+    // Work buffers per coil. Fused x pass: A [Nx, Ny, Mz, L] (outer axis
+    // transformed; unused in 2D) and B [Nx, My, Mz, L] (the fused pass works in
+    // place). Separate passes: two ping-pong grids [Mx, My, Mz, L].
+    const mr_dims &d = p->dims;
+    size_t a_per, b_per;
+    if (p->xbd_g > 0) {
+        a_per = (cfg->n_spatial == 2) ? 0 : (size_t)d.n[0] * d.n[1] * d.m[2] * cfg->L;
+        b_per = (size_t)d.n[0] * d.m[1] * d.m[2] * cfg->L;
+    } else {
+        a_per = b_per = (size_t)d.pad_size * cfg->L;
+    }
     // Coil group: as many coils as fit in mem_fraction of the free memory
-    // (MR_AHA_HIP_COIL_GROUP caps it). Two ping-pong grids of P*L*G values.
+    // (MR_AHA_HIP_COIL_GROUP caps it).
     size_t budget = 0;
     rc = mr_mem_budget(cfg, &budget);
     if (rc != MR_SUCCESS) { delete p; return rc; }
     const long long want = mr_env_ll("MR_AHA_HIP_COIL_GROUP", cfg->n_coils);
     int group = (int)((want < 1) ? 1 : (want > cfg->n_coils ? cfg->n_coils : want));
-    const size_t per_coil = 2 * (size_t)p->dims.pad_size * cfg->L * sizeof(float2);
+    const size_t per_coil = (a_per + b_per) * sizeof(float2);
     const size_t inv_bytes = cfg->is_masked ? (size_t)p->dims.pad_size * sizeof(int) : 0;
     while (group > 1 && per_coil * group + inv_bytes > budget) --group;
     for (;; --group) {
-        const size_t bytes = (size_t)p->dims.pad_size * cfg->L * group * sizeof(float2);
-        hipError_t e = hipMalloc((void **)&p->bufA, bytes);
-        if (e == hipSuccess) e = hipMalloc((void **)&p->bufB, bytes);
+        const size_t abytes = a_per * group * sizeof(float2);
+        const size_t bbytes = b_per * group * sizeof(float2);
+        hipError_t e = hipSuccess;
+        if (abytes) e = hipMalloc((void **)&p->bufA, abytes);
+        if (e == hipSuccess) e = hipMalloc((void **)&p->bufB, bbytes);
         if (e == hipSuccess && inv_bytes) e = hipMalloc((void **)&p->d_inv, inv_bytes);
         if (e == hipSuccess) break;
         (void)hipGetLastError();
@@ -595,7 +788,7 @@ static int mr_plan_create_impl(const mr_config *cfg, mr_aha_plan *out)
         if (group == 1) {
             delete p;
             MR_FAIL(MR_ERR_ALLOC, "failed to allocate %zu bytes of work buffers for one coil: %s",
-                    2 * bytes + inv_bytes, hipGetErrorString(e));
+                    abytes + bbytes + inv_bytes, hipGetErrorString(e));
         }
     }
     p->group = group;
@@ -608,6 +801,12 @@ static int mr_plan_create_impl(const mr_config *cfg, mr_aha_plan *out)
 }
 
 int mr_aha_plan_group(mr_aha_plan p) { return p->group; }
+extern "C" MR_EXPORT int mr_aha_plan_xbd_group(mr_aha_plan p)
+{
+    mr_clear_error();
+    MR_REQUIRE(p != nullptr, "mr_aha_plan_xbd_group: plan is NULL");
+    return p->xbd_g;
+}
 
 extern "C" MR_EXPORT int mr_aha_plan_create(const mr_config *cfg, mr_aha_plan *plan_out)
 {
@@ -674,6 +873,76 @@ static int mr_blockdiag(mr_aha_plan p, float2 *K, int nc, const float2 *ker)
     return MR_SUCCESS;
 }
 
+// x-last pipeline for one coil group (coils coil0 .. coil0 + nc - 1):
+//   Fo   outer axis (z; y in 2D): S_c x_l, zero-pad, FFT      -> A [Nx, Ny, Mz, V]
+//   Fy   y-FFT, zero-padding the Ny rows (3D and stack)      -> B [Nx, My, Mz, V]
+//   XBD  fused x-FFT, block multiply, inverse x-FFT, in place on B
+//   Iy   inverse y-FFT, keeping the Ny image rows            -> A
+//   Io   inverse outer FFT, crop, conj(S_c), coil sum         -> out
+// In 2D the outer axis is y itself: Fo writes B [Nx, My, V] and Io reads it.
+static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, const float2 *ker,
+                            float2 *out, int coil0, int nc)
+{
+    const mr_config &cfg = p->cfg;
+    const mr_dims d = p->dims;
+    hipStream_t s = p->stream;
+    const int L = cfg.L;
+    const int Nx = d.n[0], Ny = d.n[1], Nz = d.n[2];
+    const int My = d.m[1], Mz = d.m[2];
+    const long long N = d.image_size;
+    const long long V = (long long)L * nc;
+    const bool two = cfg.n_spatial == 2;
+    const long long plane = two ? (long long)Nx : (long long)Nx * Ny;
+    const int No = two ? Ny : Nz, Mo = two ? My : Mz, cno = two ? d.c_n[1] : d.c_n[2];
+    const mr_fft_len<float> &fo = two ? p->fy : p->fz;
+    int lgT = 0;
+    while (lgT < 6 && (plane % (2LL << lgT)) == 0) ++lgT;
+    float2 *A = two ? p->bufB : p->bufA, *B = p->bufB;
+    {
+        mr_zmap zm{V, lgT};
+        mr_ld_sx_z ld{x, S, zm, plane, N, No, Mo, cno, L, coil0};
+        mr_st_z st{A, zm, plane, Mo};
+        MR_TRY((mr_fft_pass<float, false>(fo, ld, st, plane * V, 0, 1 << lgT, s, "Fo")));
+        MR_TRY(mr_prof_mark(p, two ? MR_ST_F2 : MR_ST_F3));
+    }
+    if (!two) {
+        mr_ld_pad<float2, false> ld{A, Nx, My, Ny, d.c_n[1]};
+        mr_st_axis<float2, false> st{B, Nx, My};
+        MR_TRY((mr_fft_pass<float, false>(p->fy, ld, st, (long long)Nx * Mz * V, 0, 16, s, "Fy")));
+        MR_TRY(mr_prof_mark(p, MR_ST_F2));
+    }
+    {
+        const bool masked = cfg.is_masked != 0;
+        const long long kstride = masked ? (long long)cfg.num_masked : d.pad_size;
+        const long long R = (long long)My * Mz;
+        const int G = p->xbd_g;
+        const int ncg = (nc + G - 1) / G;
+        const long long blocks = R * ncg;
+        if (blocks > 0x7fffffffLL)
+            MR_FAIL(MR_ERR_UNSUPPORTED, "fused x pass: %lld workgroups exceed the launch limit",
+                    blocks);
+        mr_xbd_fn fn = masked ? mr_xbd_pick<true>(L) : mr_xbd_pick<false>(L);
+        hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(MR_XBD_THREADS), p->xbd_lds, s, B, ker,
+                           p->d_inv, kstride, d, p->fx, p->nxdiv, R, G, nc, ncg);
+        MR_LAUNCH_CHECK("mr_k_xbd");
+        MR_TRY(mr_prof_mark(p, MR_ST_BD));
+    }
+    if (!two) {
+        mr_ld_axis<float2, false> ld{B, Nx, My};
+        mr_st_crop<float2, false> st{A, Nx, My, Ny, d.c_n[1]};
+        MR_TRY((mr_fft_pass<float, true>(p->fy, ld, st, (long long)Nx * Mz * V, 0, 16, s, "Iy")));
+        MR_TRY(mr_prof_mark(p, MR_ST_I2));
+    }
+    {
+        mr_zmap zm{L, lgT};
+        mr_ld_z_coil ld{A, zm, plane, Mo, L, nc};
+        mr_st_z_conj_sum st{out, S, zm, plane, N, No, Mo, cno, coil0, coil0 > 0 ? 1 : 0};
+        MR_TRY((mr_fft_pass<float, true>(fo, ld, st, plane * L, (size_t)No, 1 << lgT, s, "Io")));
+        MR_TRY(mr_prof_mark(p, two ? MR_ST_I2 : MR_ST_I3));
+    }
+    return MR_SUCCESS;
+}
+
 static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, const float2 *ker,
                            const int64_t *mask, float2 *out)
 {
@@ -707,13 +976,17 @@ static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, cons
 
     for (int coil0 = 0; coil0 < cfg.n_coils; coil0 += p->group) {
         const int nc = (cfg.n_coils - coil0 < p->group) ? cfg.n_coils - coil0 : p->group;
+        if (p->xbd_g > 0) {
+            MR_TRY(mr_execute_xlast(p, x, S, ker, out, coil0, nc));
+            continue;
+        }
         const long long V = (long long)L * nc;
         float2 *A = p->bufA, *B = p->bufB;
 
         if (zfirst) {
             // Z1: S_c x_l, z-FFT on the image -> A [Nx, Ny, Nz(kz), V]
             mr_zmap zm{V, lgTXZ};
-            mr_ld_sx_z ld{x, S, zm, NxNy, N, Nz, d.c_n[2], L, coil0};
+            mr_ld_sx_z ld{x, S, zm, NxNy, N, Nz, Nz, d.c_n[2], L, coil0};
             mr_st_z st{A, zm, NxNy, Nz};
             MR_TRY((mr_fft_pass<float, false>(p->fz, ld, st, NxNy * V, 0, 1 << lgTXZ, s, "Z1")));
             MR_TRY(mr_prof_mark(p, MR_ST_F3));
@@ -776,7 +1049,7 @@ static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, cons
             // Z2: inverse z-FFT, conj(S_c), coil sum -> out
             mr_zmap zm{L, lgTXZ};
             mr_ld_z_coil ld{src, zm, NxNy, Nz, L, nc};
-            mr_st_z_conj_sum st{out, S, zm, NxNy, N, Nz, d.c_n[2], coil0, coil0 > 0 ? 1 : 0};
+            mr_st_z_conj_sum st{out, S, zm, NxNy, N, Nz, Nz, d.c_n[2], coil0, coil0 > 0 ? 1 : 0};
             MR_TRY((mr_fft_pass<float, true>(p->fz, ld, st, NxNy * L, (size_t)Nz, 1 << lgTXZ, s, "Z2")));
             MR_TRY(mr_prof_mark(p, MR_ST_I3));
         } else {

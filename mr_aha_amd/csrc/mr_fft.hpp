@@ -453,12 +453,12 @@ mr_fft_axis_kernel(mr_fft_len<R> f, Load ld, Store st, long long npencil, int lg
 #define MR_FFT_EMAX 6144
 #define MR_FFT_FAST_THREADS 512
 
-template <int RR, bool INV, typename V>
+// NT = workgroup size, EMAX = bound on tp * n (sizes the register stage).
+template <int RR, bool INV, typename V, int NT = MR_FFT_FAST_THREADS, int EMAX = MR_FFT_EMAX>
 __device__ __forceinline__ void mr_stage_reg(V *A, const V *tw, int n, int ns, int tp, int Ns,
                                              const mr_fdiv &mdv, const mr_fdiv &nsdv)
 {
-    constexpr int NT = MR_FFT_FAST_THREADS;
-    constexpr int KMAX = (MR_FFT_EMAX / RR + NT - 1) / NT;
+    constexpr int KMAX = (EMAX / RR + NT - 1) / NT;
     const int m = n / RR;
     const int stride = n / (Ns * RR);
     const int nb = tp * m;

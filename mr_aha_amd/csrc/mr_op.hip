@@ -362,3 +362,11 @@ extern "C" MR_EXPORT int mr_op_profile(mr_op op, double *ms, int *calls, int n)
     if (!op->plan) MR_FAIL(MR_ERR_UNSUPPORTED, "mr_op_profile: only Toeplitz operators are profiled");
     return mr_aha_plan_profile(op->plan, ms, calls, n);
 }
+
+extern "C" MR_EXPORT int mr_op_xbd_group(mr_op op)
+{
+    mr_clear_error();
+    MR_REQUIRE(op != nullptr, "mr_op_xbd_group: op is NULL");
+    if (!op->plan) return 0;
+    return mr_aha_plan_xbd_group(op->plan);
+}

@@ -144,6 +144,59 @@ def test_coil_group_invariance(mr, monkeypatch):
     assert rel(outs[0], outs[2]) < 1e-6 and rel(outs[1], outs[2]) < 1e-6
 
 
+def _xbd_group(mr, cfg):
+    """Coils per workgroup of the fused x pass a plan for cfg would use (0: separate passes)."""
+    from mr_aha_amd import _lib
+    plan = ctypes.c_void_p()
+    _lib.check(_lib.lib.mr_aha_plan_create(ctypes.byref(cfg), ctypes.byref(plan)), "plan_create")
+    try:
+        g = _lib.lib.mr_aha_plan_xbd_group(plan)
+        assert g >= 0
+        return g
+    finally:
+        _lib.lib.mr_aha_plan_destroy(plan)
+
+
+@pytest.mark.parametrize("name", ["2d_50x40_L3_c3", "3d_24x20x16_L4_c2", "3d_25x21x15_L3_c2_odd",
+                                  "stack_40x36x10_L3_c3", "stack_31x36x9_L4_c2_odd",
+                                  "radial_128_L4_C4", "2d_33x27_L1_c2_odd"])
+@pytest.mark.parametrize("masked", [False, True])
+def test_fused_and_separate_routes(mr, monkeypatch, name, masked):
+    """The fused x pass (x-FFT, block multiply, inverse x-FFT per k-space row)
+    and the separate FFT/block passes agree with the FP64 apply and with each
+    other, for 1, 2 and all coils per fused workgroup, both triangles. Lengths
+    with a prime factor above 7 (the 33x27 case) run the separate passes
+    whatever is asked."""
+    from oracle import toeplitz
+    print(f"[routes] {name} masked={masked}", flush=True)
+    g, traj, phi2, x, S = case(name)
+    dx, dS = dev(x), dev(S)
+    for tri in ("lower", "upper"):
+        cfg = cfg_for(mr, g, triangulation=tri, is_masked=masked)
+        mask = mr.calc_mask(cfg, dev(traj)) if masked else None
+        K = mr.calc_toep(cfg, dev(phi2), dev(traj), mask)
+        ref = toeplitz.apply(g, x, S, host(K).astype(np.complex128), upper=(tri == "upper"),
+                             mask=host(mask) if masked else None)
+        outs = {}
+        for route, G in (("separate", None), ("fused", "1"), ("fused", "2"), ("fused", "64")):
+            monkeypatch.setenv("MR_AHA_HIP_XBD", "0" if route == "separate" else "1")
+            if G is None:
+                monkeypatch.delenv("MR_AHA_HIP_XBD_G", raising=False)
+            else:
+                monkeypatch.setenv("MR_AHA_HIP_XBD_G", G)
+            got = _xbd_group(mr, cfg)
+            if route == "separate":
+                assert got == 0
+            elif name != "2d_33x27_L1_c2_odd":
+                assert got == min(int(G), g.n_coils), (route, G, got)
+            y = host(mr.aha_toeplitz(cfg, dx, dS, K, mask))
+            assert rel(y, ref) < 1e-5, (route, G, tri)
+            outs[(route, G)] = y
+        base = outs[("separate", None)]
+        for k, y in outs.items():
+            assert rel(y, base) < 1e-6, (k, tri)
+
+
 @pytest.mark.parametrize("L", [1, 2, 5, 7, 12, 13, 16, 17, 20])
 def test_block_ranks(mr, L):
     """Every rank (specialised kernels up to 16, the any-rank kernel above)
