@@ -481,19 +481,78 @@ __device__ __forceinline__ void mr_xbd_fft(float2 *A, const mr_fft_len<float> &f
         const int RR = f.radix[s];
         const mr_fdiv &md = f.mdiv[s], &nd = f.nsdiv[s];
         switch (RR) {
-        case 2: mr_stage_reg<2, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tw, f.n, f.n, tp, Ns, md, nd); break;
-        case 3: mr_stage_reg<3, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tw, f.n, f.n, tp, Ns, md, nd); break;
-        case 4: mr_stage_reg<4, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tw, f.n, f.n, tp, Ns, md, nd); break;
-        case 5: mr_stage_reg<5, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tw, f.n, f.n, tp, Ns, md, nd); break;
-        case 7: mr_stage_reg<7, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tw, f.n, f.n, tp, Ns, md, nd); break;
-        default: mr_stage_reg<8, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tw, f.n, f.n, tp, Ns, md, nd); break;
+        case 2: mr_stage_reg<2, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tws, f.n, f.n, tp, Ns, md, nd); break;
+        case 3: mr_stage_reg<3, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tws, f.n, f.n, tp, Ns, md, nd); break;
+        case 4: mr_stage_reg<4, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tws, f.n, f.n, tp, Ns, md, nd); break;
+        case 5: mr_stage_reg<5, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tws, f.n, f.n, tp, Ns, md, nd); break;
+        case 7: mr_stage_reg<7, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tws, f.n, f.n, tp, Ns, md, nd); break;
+        default: mr_stage_reg<8, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tws, f.n, f.n, tp, Ns, md, nd); break;
         }
         Ns *= RR;
     }
 }
 
+// Block multiply of one location with its packed columns read from global
+// memory, in place on the L values a[0], a[ms], ... (LDS). Columns are walked
+// so that output c is final once column c is done (lower: ascending c, since
+// column c holds rows r >= c and the mirrors of earlier columns have already
+// reached y_c; upper: descending c, by symmetry), and v_c is not read by any
+// later column: y_c is stored over v_c at once. The inputs v_r are read from
+// LDS where they are used (their slots are only overwritten once dead), so
+// the registers hold the L partial outputs and two columns of kernel entries:
+// the next column's loads are issued before the current one is multiplied.
+template <int L, bool UPPER>
+__device__ __forceinline__ void mr_bd_walk_g(float2 *a, int ms, const float2 *__restrict__ bp,
+                                             long long ks)
+{
+    float2 y[L], kk[2][L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) y[l] = make_float2(0.f, 0.f);
+    // packed offset and length of the i-th column walked
+    auto col = [](int i, int &c, int &t0, int &len) {
+        c = UPPER ? L - 1 - i : i;
+        t0 = UPPER ? c * (c + 1) / 2 : c * L - c * (c - 1) / 2;
+        len = UPPER ? c + 1 : L - c;
+    };
+    {
+        int c, t0, len;
+        col(0, c, t0, len);
+        const float2 *p = bp + ks * t0;
+#pragma unroll
+        for (int j = 0; j < L; ++j)
+            if (j < len) kk[0][j] = p[ks * j];
+    }
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        int c, t0, len;
+        col(i, c, t0, len);
+        if (i + 1 < L) {
+            int c1, t1, len1;
+            col(i + 1, c1, t1, len1);
+            const float2 *p = bp + ks * t1;
+#pragma unroll
+            for (int j = 0; j < L; ++j)
+                if (j < len1) kk[(i + 1) & 1][j] = p[ks * j];
+        }
+        asm volatile("" ::: "memory");
+        const int r0 = UPPER ? 0 : c;
+        const float2 vc = a[c * ms];
+#pragma unroll
+        for (int j = 0; j < L; ++j) {
+            if (j < len) {
+                const int r = r0 + j;
+                const float2 e = kk[i & 1][j];
+                const float2 vr = (r == c) ? vc : a[r * ms];
+                y[c] = cadd(y[c], cmul(e, vr));
+                if (r != c) y[r] = cadd(y[r], cmulc(e, vc));
+            }
+        }
+        a[c * ms] = y[c];
+    }
+}
+
 template <int L, bool MASKED>
-__global__ void __launch_bounds__(MR_XBD_THREADS)
+__global__ void __launch_bounds__(MR_XBD_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
 mr_k_xbd(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__restrict__ inv,
          long long kstride, mr_dims d, mr_fft_len<float> fx, mr_fdiv nxdiv, long long R, int G,
          int ncoil, int ncg)
@@ -535,17 +594,14 @@ mr_k_xbd(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__re
             b = rowc + kx;
         }
         float2 *a = A + (size_t)g * L * Mx + e;
-        float2 v[1][L], y[1][L];
+        if (b < 0) {
 #pragma unroll
-        for (int l = 0; l < L; ++l) v[0][l] = a[l * Mx];
-        if (b >= 0) {
-            mr_bd_walk<L, 1>(v, y, ker + b, kstride, d.tri_upper);
+            for (int l = 0; l < L; ++l) a[l * Mx] = make_float2(0.f, 0.f);
+        } else if (d.tri_upper) {
+            mr_bd_walk_g<L, true>(a, Mx, ker + b, kstride);
         } else {
-#pragma unroll
-            for (int l = 0; l < L; ++l) y[0][l] = make_float2(0.f, 0.f);
+            mr_bd_walk_g<L, false>(a, Mx, ker + b, kstride);
         }
-#pragma unroll
-        for (int l = 0; l < L; ++l) a[l * Mx] = y[0][l];
     }
     __syncthreads();
     mr_xbd_fft<true>(A, fx, tp);

@@ -91,6 +91,12 @@ template <typename R> struct mr_fft_len {
     mr_fdiv nsdiv[MR_FFT_MAXST];   // product of the radices before stage s
     mr_fdiv ndiv;                  // n
     const typename cvec<R>::T *tw;  // tw[q] = exp(-2 pi i q / n), q < n (device)
+    // Stage-major table (device, n entries): the twiddle of butterfly input r
+    // at offset k < Ns of stage (R, Ns) is tws[r * Ns + k - 1] =
+    // exp(-2 pi i k r / (Ns R)), 1 <= r < R. Consecutive k are consecutive
+    // entries, so a wavefront's reads are coalesced (global) or free of bank
+    // conflicts (LDS); the stages' ranges [Ns - 1, Ns R - 1) tile [0, n - 1).
+    const typename cvec<R>::T *tws;
 };
 
 template <typename R> inline void mr_fft_fill_divs(mr_fft_len<R> *f)
@@ -260,7 +266,6 @@ __device__ __forceinline__ void mr_stage(const V *__restrict__ A, V *__restrict_
                                          const mr_fdiv &nsdv)
 {
     const int m = n / RR;
-    const int stride = n / (Ns * RR);
     for (int f = threadIdx.x; f < tp * m; f += blockDim.x) {
         const int p = (int)mr_fdiv_q(mdv, (unsigned)f);
         const int j = f - p * m;
@@ -273,7 +278,7 @@ __device__ __forceinline__ void mr_stage(const V *__restrict__ A, V *__restrict_
         if (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < RR; ++r) {
-                V w = tw[k * r * stride];
+                V w = tw[r * Ns + k - 1];
                 v[r] = INV ? cmulc(w, v[r]) : cmul(w, v[r]);
             }
         }
@@ -285,7 +290,8 @@ __device__ __forceinline__ void mr_stage(const V *__restrict__ A, V *__restrict_
 }
 
 // Any radix: each thread evaluates one output of one butterfly as a length-R
-// dot product; the combined stage and DFT exponent is r (k + Ns q) stride.
+// dot product; the combined stage and DFT exponent is r (k + Ns q) stride,
+// looked up in the natural-order table tw (global memory).
 template <bool INV, typename V>
 __device__ __forceinline__ void mr_stage_any(const V *__restrict__ A, V *__restrict__ B,
                                              const V *tw, int n, int ns, int tp, int Ns, int RR)
@@ -329,7 +335,7 @@ __device__ typename cvec<R>::T *mr_fft_lds(const mr_fft_len<R> &f, typename cvec
         case 5: mr_stage<5, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
         case 7: mr_stage<7, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
         case 8: mr_stage<8, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
-        default: mr_stage_any<INV, V>(A, B, tw, f.n, ns, tp, Ns, RR); break;
+        default: mr_stage_any<INV, V>(A, B, f.tw, f.n, ns, tp, Ns, RR); break;
         }
         __syncthreads();
         V *t = A; A = B; B = t;
@@ -425,7 +431,7 @@ mr_fft_axis_kernel(mr_fft_len<R> f, Load ld, Store st, long long npencil, int lg
     V *acc = B0 + (size_t)TX * ns;
     const long long p0 = mr_xcd_block(blockIdx.x, gridDim.x) * TX;
     const int tp = (int)((npencil - p0 < TX) ? (npencil - p0) : TX);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = f.tw[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = f.tws[i];
     const int nsub = ld.nsub();
     for (int sub = 0; sub < nsub; ++sub) {
         if (threadIdx.x < tp) {
@@ -460,7 +466,6 @@ __device__ __forceinline__ void mr_stage_reg(V *A, const V *tw, int n, int ns, i
 {
     constexpr int KMAX = (EMAX / RR + NT - 1) / NT;
     const int m = n / RR;
-    const int stride = n / (Ns * RR);
     const int nb = tp * m;
     V v[KMAX][RR];
 #pragma unroll
@@ -484,7 +489,7 @@ __device__ __forceinline__ void mr_stage_reg(V *A, const V *tw, int n, int ns, i
             if (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < RR; ++r) {
-                    const V w = tw[kk * r * stride];
+                    const V w = tw[r * Ns + kk - 1];
                     v[k][r] = INV ? cmulc(w, v[k][r]) : cmul(w, v[k][r]);
                 }
             }
@@ -513,7 +518,7 @@ mr_fft_axis_fast(mr_fft_len<R> f, Load ld, Store st, long long npencil, int lgTX
     V *acc = A + (size_t)TX * ns;
     const long long p0 = mr_xcd_block(blockIdx.x, gridDim.x) * TX;
     const int tp = (int)((npencil - p0 < TX) ? (npencil - p0) : TX);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = f.tw[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = f.tws[i];
     const int nsub = ld.nsub();
     for (int sub = 0; sub < nsub; ++sub) {
         if (threadIdx.x < tp) {

@@ -49,12 +49,26 @@ template <typename R> int mr_fft_get(int n, mr_fft_len<R> *out)
                 h[q].y = sn[quad];
             }
         }
+        // stage-major copy: entry r * Ns + k - 1 of stage (R, Ns) is
+        // w_n^(k r n / (Ns R)); the stages tile [0, n - 1)
+        std::vector<V> hs(n, h[0]);
+        int Ns = 1;
+        for (int s = 0; s < out->nst; ++s) {
+            const int rad = out->radix[s];
+            const int stride = n / (Ns * rad);
+            for (int r = 1; r < rad; ++r)
+                for (int k = 0; k < Ns; ++k) hs[r * Ns + k - 1] = h[(long long)k * r * stride];
+            Ns *= rad;
+        }
         void *d = nullptr;
-        MR_HIP_CHECK(hipMalloc(&d, n * sizeof(V)));
+        MR_HIP_CHECK(hipMalloc(&d, 2 * n * sizeof(V)));
         MR_HIP_CHECK(hipMemcpy(d, h.data(), n * sizeof(V), hipMemcpyHostToDevice));
+        MR_HIP_CHECK(hipMemcpy(reinterpret_cast<V *>(d) + n, hs.data(), n * sizeof(V),
+                               hipMemcpyHostToDevice));
         it = g_tw.emplace(key, d).first;
     }
     out->tw = reinterpret_cast<const V *>(it->second);
+    out->tws = out->tw + n;
     mr_fft_fill_divs(out);
     return MR_SUCCESS;
 }
