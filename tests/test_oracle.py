@@ -59,6 +59,30 @@ def test_centering_maps(M):
         assert len(set(p.tolist())) == N
 
 
+def test_centering_literals():
+    """Hand-worked values of the maps of src/cuda/mr_internal.cuh:119-139 and
+    src/cuda/mr_kernels.cu:111-128: pad j = (x - floor(N/2)) mod M,
+    centered k = (j + floor(M/2)) mod M."""
+    assert cv.pad_index(0, 5, 8) == 6 and cv.pad_index(2, 5, 8) == 0 and cv.pad_index(4, 5, 8) == 2
+    assert cv.pad_index(0, 4, 7) == 5 and cv.pad_index(3, 4, 7) == 1
+    assert cv.nat2cent(0, 8) == 4 and cv.nat2cent(7, 8) == 3 and cv.nat2cent(0, 7) == 3
+    assert cv.cent2nat(0, 8) == 4 and cv.cent2nat(3, 7) == 0 and cv.cent2nat(6, 7) == 3
+
+
+def test_cpu_baseline_matches_apply():
+    """The timed CPU baseline computes the oracle's apply (complex64 rounding)."""
+    from oracle import cpu_baseline as cb
+    for name in ("2d_50x40_L3_c3", "stack_31x36x9_L4_c2_odd", "3d_25x21x15_L3_c2_odd"):
+        g, traj, phi2, x, S = gen.reference_case(name)
+        for up in (False, True):
+            K = toeplitz.calc_kernel(g, phi2, traj, upper=up)
+            ref = toeplitz.apply(g, x, S, K, upper=up)
+            y = cb.toeplitz_apply_coils(list(g.n), list(g.M), g.L, x.astype(np.complex64),
+                                        S.astype(np.complex64), K.astype(np.complex64), up,
+                                        range(g.n_coils), 2)
+            assert rel(y, ref) < 1e-5, (name, up)
+
+
 @pytest.mark.parametrize("M", [4, 7, 64])
 def test_dirichlet_closed_form(M):
     # D_M(t) = sum_{d=-floor(M/2)}^{ceil(M/2)-1} e^{i d t}
