@@ -492,18 +492,51 @@ __device__ __forceinline__ void mr_xbd_fft(float2 *A, const mr_fft_len<float> &f
     }
 }
 
-// Block multiply of one location with its packed columns read from global
-// memory, in place on the L values a[0], a[ms], ... (LDS). Columns are walked
-// so that output c is final once column c is done (lower: ascending c, since
-// column c holds rows r >= c and the mirrors of earlier columns have already
-// reached y_c; upper: descending c, by symmetry), and v_c is not read by any
-// later column: y_c is stored over v_c at once. The inputs v_r are read from
-// LDS where they are used (their slots are only overwritten once dead), so
-// the registers hold the L partial outputs and two columns of kernel entries:
-// the next column's loads are issued before the current one is multiplied.
+// Raw buffer access (gfx9 resource word 3 = 0x00020000): the base address and
+// the byte range sit in scalar registers, a lane supplies a 32-bit byte
+// offset, and reads past the range return zero. A wavefront-uniform base
+// (a kernel column, an image pencil) then costs no vector instructions for
+// addressing, where a 64-bit pointer per lane costs two or three per access.
+typedef unsigned int mr_u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t mr_rsrc(const void *base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+__device__ __forceinline__ float2 mr_bload(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                          unsigned soff = 0)
+{
+    const mr_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, (int)soff, 0);
+    return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+
+__device__ __forceinline__ void mr_bstore(__amdgpu_buffer_rsrc_t r, unsigned off, float2 v)
+{
+    mr_u32x2 u;
+    u.x = __float_as_uint(v.x);
+    u.y = __float_as_uint(v.y);
+    __builtin_amdgcn_raw_buffer_store_b64(u, r, (int)off, 0, 0);
+}
+
+// Block multiply of one location, in place on the L values a[0], a[ms], ...
+// (LDS). Packed entry t of the location is read at byte offset boff of
+// kernel column t (ker + t * ks, colbytes long); an offset past the column
+// (a location outside the mask) reads zeros and yields y = 0. One resource
+// serves a walked column: its entries differ by the scalar offset j*ks*8,
+// which the range check leaves out, so the check stays on boff alone. Columns are
+// walked so that output c is final once column c is done (lower: ascending
+// c, since column c holds rows r >= c and the mirrors of earlier columns have
+// already reached y_c; upper: descending c, by symmetry), and v_c is not
+// read by any later column: y_c is stored over v_c at once. The inputs v_r
+// are read from LDS where they are used (their slots are only overwritten
+// once dead), so the registers hold the L partial outputs and two columns of
+// kernel entries: the next column's loads are issued before the current one
+// is multiplied.
 template <int L, bool UPPER>
-__device__ __forceinline__ void mr_bd_walk_g(float2 *a, int ms, const float2 *__restrict__ bp,
-                                             long long ks)
+__device__ __forceinline__ void mr_bd_walk_g(float2 *a, int ms, const float2 *ker, long long ks,
+                                             unsigned colbytes, unsigned boff)
 {
     float2 y[L], kk[2][L];
 #pragma unroll
@@ -517,10 +550,10 @@ __device__ __forceinline__ void mr_bd_walk_g(float2 *a, int ms, const float2 *__
     {
         int c, t0, len;
         col(0, c, t0, len);
-        const float2 *p = bp + ks * t0;
+        const __amdgpu_buffer_rsrc_t rs = mr_rsrc(ker + ks * t0, colbytes);
 #pragma unroll
         for (int j = 0; j < L; ++j)
-            if (j < len) kk[0][j] = p[ks * j];
+            if (j < len) kk[0][j] = mr_bload(rs, boff, (unsigned)(ks * 8 * j));
     }
 #pragma unroll
     for (int i = 0; i < L; ++i) {
@@ -529,10 +562,10 @@ __device__ __forceinline__ void mr_bd_walk_g(float2 *a, int ms, const float2 *__
         if (i + 1 < L) {
             int c1, t1, len1;
             col(i + 1, c1, t1, len1);
-            const float2 *p = bp + ks * t1;
+            const __amdgpu_buffer_rsrc_t rs = mr_rsrc(ker + ks * t1, colbytes);
 #pragma unroll
             for (int j = 0; j < L; ++j)
-                if (j < len1) kk[(i + 1) & 1][j] = p[ks * j];
+                if (j < len1) kk[(i + 1) & 1][j] = mr_bload(rs, boff, (unsigned)(ks * 8 * j));
         }
         asm volatile("" ::: "memory");
         const int r0 = UPPER ? 0 : c;
@@ -557,6 +590,7 @@ mr_k_xbd(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__re
          long long kstride, mr_dims d, mr_fft_len<float> fx, mr_fdiv nxdiv, long long R, int G,
          int ncoil, int ncg)
 {
+    constexpr int NW = MR_XBD_THREADS / 64;
     extern __shared__ __align__(16) unsigned char mr_smem[];
     float2 *A = reinterpret_cast<float2 *>(mr_smem);
     const int Nx = d.n[0], Mx = d.m[0], cnx = d.c_n[0];
@@ -565,15 +599,22 @@ mr_k_xbd(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__re
     const int c0 = (int)(q - r * ncg) * G;
     const int gn = (ncoil - c0 < G) ? ncoil - c0 : G;
     const int tp = gn * L;
-    // pencil pl = g*L + l of the workgroup is volume v = L*c0 + pl
-    float2 *Yr = Y + (long long)Nx * r + (long long)Nx * R * L * c0;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // pencil pl = g*L + l of the workgroup is volume v = L*c0 + pl; every
+    // wavefront loads and stores whole pencils, so a pencil's base is uniform
+    const float2 *Yr = Y + (long long)Nx * r + (long long)Nx * R * L * c0;
     const long long pstride = (long long)Nx * R;
+    const unsigned rowbytes = (unsigned)Nx * 8u;
 
-    for (int i = threadIdx.x; i < tp * Mx; i += MR_XBD_THREADS) {
-        const int pl = (int)mr_fdiv_q(fx.ndiv, (unsigned)i), e = i - pl * Mx;
-        int s = e + cnx;
-        if (s >= Mx) s -= Mx;
-        A[i] = (s < Nx) ? Yr[pstride * pl + s] : make_float2(0.f, 0.f);
+    for (int pl = wave; pl < tp; pl += NW) {
+        const __amdgpu_buffer_rsrc_t rs = mr_rsrc(Yr + pstride * pl, rowbytes);
+        float2 *Ap = A + pl * Mx;
+        for (int e = lane; e < Mx; e += 64) {
+            int s = e + cnx;
+            if (s >= Mx) s -= Mx;
+            Ap[e] = mr_bload(rs, (unsigned)s * 8u);   // s >= Nx reads zero: the padding
+        }
     }
     __syncthreads();
     mr_xbd_fft<false>(A, fx, tp);
@@ -583,6 +624,7 @@ mr_k_xbd(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__re
     int ky = jy + d.c_m[1]; if (ky >= My) ky -= My;
     int kz = jz + d.c_m[2]; if (kz >= d.m[2]) kz -= d.m[2];
     const long long rowc = (long long)Mx * (ky + (long long)My * kz);
+    const unsigned colbytes = (unsigned)(kstride * 8);
     for (int w = threadIdx.x; w < gn * Mx; w += MR_XBD_THREADS) {
         const int g = (int)mr_fdiv_q(fx.ndiv, (unsigned)w), e = w - g * Mx;
         long long b;
@@ -593,24 +635,24 @@ mr_k_xbd(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__re
             if (kx >= Mx) kx -= Mx;
             b = rowc + kx;
         }
+        const unsigned boff = (b < 0) ? 0xFFFFFFF0u : (unsigned)(b * 8);
         float2 *a = A + (size_t)g * L * Mx + e;
-        if (b < 0) {
-#pragma unroll
-            for (int l = 0; l < L; ++l) a[l * Mx] = make_float2(0.f, 0.f);
-        } else if (d.tri_upper) {
-            mr_bd_walk_g<L, true>(a, Mx, ker + b, kstride);
-        } else {
-            mr_bd_walk_g<L, false>(a, Mx, ker + b, kstride);
-        }
+        if (d.tri_upper)
+            mr_bd_walk_g<L, true>(a, Mx, ker, kstride, colbytes, boff);
+        else
+            mr_bd_walk_g<L, false>(a, Mx, ker, kstride, colbytes, boff);
     }
     __syncthreads();
     mr_xbd_fft<true>(A, fx, tp);
 
-    for (int i = threadIdx.x; i < tp * Nx; i += MR_XBD_THREADS) {
-        const int pl = (int)mr_fdiv_q(nxdiv, (unsigned)i), s = i - pl * Nx;
-        int e = s - cnx;
-        if (e < 0) e += Mx;
-        Yr[pstride * pl + s] = A[pl * Mx + e];
+    for (int pl = wave; pl < tp; pl += NW) {
+        const __amdgpu_buffer_rsrc_t rs = mr_rsrc(Yr + pstride * pl, rowbytes);
+        const float2 *Ap = A + pl * Mx;
+        for (int s = lane; s < Nx; s += 64) {
+            int e = s - cnx;
+            if (e < 0) e += Mx;
+            mr_bstore(rs, (unsigned)s * 8u, Ap[e]);
+        }
     }
 }
 
@@ -770,6 +812,11 @@ static int mr_xbd_setup(mr_aha_plan p)
     const long long per_elems = (long long)L * Mx;
     const size_t per = (size_t)per_elems * sizeof(float2);
     if (per_elems > MR_XBD_EMAX) return MR_SUCCESS;
+    // kernel entries are addressed with 32-bit byte offsets: one column, plus
+    // up to L - 1 columns of scalar offset; one offset past the column marks
+    // a location outside the mask
+    const long long kcol = cfg.is_masked ? (long long)cfg.num_masked : p->dims.pad_size;
+    if (kcol * 8 * L > 0xFFFFFF00LL) return MR_SUCCESS;
     int dev = 0, maxlds = 0;
     MR_HIP_CHECK(hipGetDevice(&dev));
     MR_HIP_CHECK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
