@@ -22,6 +22,7 @@
 
 #include "mr_hip.hpp"
 #include "mr_aha/mr_aha_ext.h"
+#include "mr_xbd.hpp"
 
 #include <cstdlib>
 #include <cstring>
@@ -470,8 +471,6 @@ template <bool MASKED> static mr_bd_fn mr_bd_pick(int L, int TP)
 // values plus the kernel row. Twiddles are read from the global table (L1
 // resident) so LDS holds exactly G*L*Mx values and two 80 KB workgroups can
 // share a CU.
-#define MR_XBD_THREADS 512
-#define MR_XBD_EMAX 16384
 
 template <bool INV>
 __device__ __forceinline__ void mr_xbd_fft(float2 *A, const mr_fft_len<float> &f, int tp)
@@ -489,98 +488,6 @@ __device__ __forceinline__ void mr_xbd_fft(float2 *A, const mr_fft_len<float> &f
         default: mr_stage_reg<8, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tws, f.n, f.n, tp, Ns, md, nd); break;
         }
         Ns *= RR;
-    }
-}
-
-// Raw buffer access (gfx9 resource word 3 = 0x00020000): the base address and
-// the byte range sit in scalar registers, a lane supplies a 32-bit byte
-// offset, and reads past the range return zero. A wavefront-uniform base
-// (a kernel column, an image pencil) then costs no vector instructions for
-// addressing, where a 64-bit pointer per lane costs two or three per access.
-typedef unsigned int mr_u32x2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t mr_rsrc(const void *base, unsigned bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
-                                             0x00020000);
-}
-
-__device__ __forceinline__ float2 mr_bload(__amdgpu_buffer_rsrc_t r, unsigned off,
-                                          unsigned soff = 0)
-{
-    const mr_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, (int)soff, 0);
-    return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
-}
-
-__device__ __forceinline__ void mr_bstore(__amdgpu_buffer_rsrc_t r, unsigned off, float2 v)
-{
-    mr_u32x2 u;
-    u.x = __float_as_uint(v.x);
-    u.y = __float_as_uint(v.y);
-    __builtin_amdgcn_raw_buffer_store_b64(u, r, (int)off, 0, 0);
-}
-
-// Block multiply of one location, in place on the L values a[0], a[ms], ...
-// (LDS). Packed entry t of the location is read at byte offset boff of
-// kernel column t (ker + t * ks, colbytes long); an offset past the column
-// (a location outside the mask) reads zeros and yields y = 0. One resource
-// serves a walked column: its entries differ by the scalar offset j*ks*8,
-// which the range check leaves out, so the check stays on boff alone. Columns are
-// walked so that output c is final once column c is done (lower: ascending
-// c, since column c holds rows r >= c and the mirrors of earlier columns have
-// already reached y_c; upper: descending c, by symmetry), and v_c is not
-// read by any later column: y_c is stored over v_c at once. The inputs v_r
-// are read from LDS where they are used (their slots are only overwritten
-// once dead), so the registers hold the L partial outputs and two columns of
-// kernel entries: the next column's loads are issued before the current one
-// is multiplied.
-template <int L, bool UPPER>
-__device__ __forceinline__ void mr_bd_walk_g(float2 *a, int ms, const float2 *ker, long long ks,
-                                             unsigned colbytes, unsigned boff)
-{
-    float2 y[L], kk[2][L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) y[l] = make_float2(0.f, 0.f);
-    // packed offset and length of the i-th column walked
-    auto col = [](int i, int &c, int &t0, int &len) {
-        c = UPPER ? L - 1 - i : i;
-        t0 = UPPER ? c * (c + 1) / 2 : c * L - c * (c - 1) / 2;
-        len = UPPER ? c + 1 : L - c;
-    };
-    {
-        int c, t0, len;
-        col(0, c, t0, len);
-        const __amdgpu_buffer_rsrc_t rs = mr_rsrc(ker + ks * t0, colbytes);
-#pragma unroll
-        for (int j = 0; j < L; ++j)
-            if (j < len) kk[0][j] = mr_bload(rs, boff, (unsigned)(ks * 8 * j));
-    }
-#pragma unroll
-    for (int i = 0; i < L; ++i) {
-        int c, t0, len;
-        col(i, c, t0, len);
-        if (i + 1 < L) {
-            int c1, t1, len1;
-            col(i + 1, c1, t1, len1);
-            const __amdgpu_buffer_rsrc_t rs = mr_rsrc(ker + ks * t1, colbytes);
-#pragma unroll
-            for (int j = 0; j < L; ++j)
-                if (j < len1) kk[(i + 1) & 1][j] = mr_bload(rs, boff, (unsigned)(ks * 8 * j));
-        }
-        asm volatile("" ::: "memory");
-        const int r0 = UPPER ? 0 : c;
-        const float2 vc = a[c * ms];
-#pragma unroll
-        for (int j = 0; j < L; ++j) {
-            if (j < len) {
-                const int r = r0 + j;
-                const float2 e = kk[i & 1][j];
-                const float2 vr = (r == c) ? vc : a[r * ms];
-                y[c] = cadd(y[c], cmul(e, vr));
-                if (r != c) y[r] = cadd(y[r], cmulc(e, vc));
-            }
-        }
-        a[c * ms] = y[c];
     }
 }
 
@@ -624,7 +531,6 @@ mr_k_xbd(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__re
     int ky = jy + d.c_m[1]; if (ky >= My) ky -= My;
     int kz = jz + d.c_m[2]; if (kz >= d.m[2]) kz -= d.m[2];
     const long long rowc = (long long)Mx * (ky + (long long)My * kz);
-    const unsigned colbytes = (unsigned)(kstride * 8);
     for (int w = threadIdx.x; w < gn * Mx; w += MR_XBD_THREADS) {
         const int g = (int)mr_fdiv_q(fx.ndiv, (unsigned)w), e = w - g * Mx;
         long long b;
@@ -635,12 +541,15 @@ mr_k_xbd(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__re
             if (kx >= Mx) kx -= Mx;
             b = rowc + kx;
         }
-        const unsigned boff = (b < 0) ? 0xFFFFFFF0u : (unsigned)(b * 8);
         float2 *a = A + (size_t)g * L * Mx + e;
-        if (d.tri_upper)
-            mr_bd_walk_g<L, true>(a, Mx, ker, kstride, colbytes, boff);
-        else
-            mr_bd_walk_g<L, false>(a, Mx, ker, kstride, colbytes, boff);
+        if (b < 0) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) a[l * Mx] = make_float2(0.f, 0.f);
+        } else if (d.tri_upper) {
+            mr_bd_walk_g<L, true>(a, Mx, ker, kstride, (unsigned)(b * 8));
+        } else {
+            mr_bd_walk_g<L, false>(a, Mx, ker, kstride, (unsigned)(b * 8));
+        }
     }
     __syncthreads();
     mr_xbd_fft<true>(A, fx, tp);
@@ -703,6 +612,7 @@ struct mr_aha_plan_s {
     int xbd_g;          // coils per workgroup of the fused x pass; 0 = separate passes
     size_t xbd_lds;     // its dynamic LDS bytes
     mr_fdiv nxdiv;
+    mr_xbd_ct_fn xbd_ct;  // kernel specialised for (Mx, L), or null: mr_k_xbd
 };
 
 static int mr_prof_mark(mr_aha_plan p, int stage)
@@ -797,13 +707,15 @@ static void mr_plan_free_buffers(mr_aha_plan p)
 
 // Decide whether this plan runs the fused x pass: rank up to 16, an x length
 // made of the unrolled radices, and one coil's padded row within the LDS of a
-// workgroup. Default G: the most coils that keep a workgroup within 80 KB of
-// LDS, so two workgroups share a CU. MR_AHA_HIP_XBD=0 selects the separate
-// passes; MR_AHA_HIP_XBD_G sets G.
+// workgroup. (Mx, L) pairs with a compile-time kernel use it. Default G: the
+// most coils that keep a workgroup within 80 KB of LDS, so two workgroups
+// share a CU. MR_AHA_HIP_XBD=0 selects the separate passes,
+// MR_AHA_HIP_XBD_CT=0 the run-time-length kernel; MR_AHA_HIP_XBD_G sets G.
 static int mr_xbd_setup(mr_aha_plan p)
 {
     p->xbd_g = 0;
     p->xbd_lds = 0;
+    p->xbd_ct = nullptr;
     const mr_config &cfg = p->cfg;
     const int L = cfg.L, Mx = p->dims.m[0];
     if (mr_env_ll("MR_AHA_HIP_XBD", 1) == 0 || L > 16) return MR_SUCCESS;
@@ -813,8 +725,7 @@ static int mr_xbd_setup(mr_aha_plan p)
     const size_t per = (size_t)per_elems * sizeof(float2);
     if (per_elems > MR_XBD_EMAX) return MR_SUCCESS;
     // kernel entries are addressed with 32-bit byte offsets: one column, plus
-    // up to L - 1 columns of scalar offset; one offset past the column marks
-    // a location outside the mask
+    // up to L - 1 columns of scalar offset
     const long long kcol = cfg.is_masked ? (long long)cfg.num_masked : p->dims.pad_size;
     if (kcol * 8 * L > 0xFFFFFF00LL) return MR_SUCCESS;
     int dev = 0, maxlds = 0;
@@ -823,22 +734,34 @@ static int mr_xbd_setup(mr_aha_plan p)
     if (per > (size_t)maxlds) return MR_SUCCESS;
     long long gmax = MR_XBD_EMAX / per_elems;
     if (gmax > (long long)(maxlds / per)) gmax = (long long)(maxlds / per);
+    const void *fn = nullptr;
+    int ct_gmax = 0;
+    mr_xbd_ct_fn ct = nullptr;
+    if (mr_env_ll("MR_AHA_HIP_XBD_CT", 1) != 0)
+        ct = mr_xbd_ct_pick(Mx, L, cfg.is_masked != 0, &ct_gmax);
+    if (ct) {
+        fn = (const void *)ct;
+        if (gmax > ct_gmax) gmax = ct_gmax;
+    } else {
+        mr_xbd_fn gf = cfg.is_masked ? mr_xbd_pick<true>(L) : mr_xbd_pick<false>(L);
+        if (!gf) return MR_SUCCESS;
+        fn = (const void *)gf;
+    }
     if (gmax > cfg.n_coils) gmax = cfg.n_coils;
     long long g = (long long)(81920 / per);
     g = mr_env_ll("MR_AHA_HIP_XBD_G", g);
     if (g > gmax) g = gmax;
     if (g < 1) g = 1;
     const size_t lds = per * (size_t)g;
-    mr_xbd_fn fn = cfg.is_masked ? mr_xbd_pick<true>(L) : mr_xbd_pick<false>(L);
-    if (!fn) return MR_SUCCESS;
     if (lds > 65536 &&
-        hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds) != hipSuccess) {
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+            hipSuccess) {
         (void)hipGetLastError();
         return MR_SUCCESS;
     }
     p->xbd_g = (int)g;
     p->xbd_lds = lds;
+    p->xbd_ct = ct;
     p->nxdiv = mr_fdiv_make((unsigned)p->dims.n[0]);
     return MR_SUCCESS;
 }
@@ -1024,10 +947,16 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
         if (blocks > 0x7fffffffLL)
             MR_FAIL(MR_ERR_UNSUPPORTED, "fused x pass: %lld workgroups exceed the launch limit",
                     blocks);
-        mr_xbd_fn fn = masked ? mr_xbd_pick<true>(L) : mr_xbd_pick<false>(L);
-        hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(MR_XBD_THREADS), p->xbd_lds, s, B, ker,
-                           p->d_inv, kstride, d, p->fx, p->nxdiv, R, G, nc, ncg);
-        MR_LAUNCH_CHECK("mr_k_xbd");
+        if (p->xbd_ct) {
+            hipLaunchKernelGGL(p->xbd_ct, dim3((unsigned)blocks), dim3(MR_XBD_THREADS), p->xbd_lds,
+                               s, B, ker, p->d_inv, kstride, d, p->fx.tws, R, G, nc, ncg);
+            MR_LAUNCH_CHECK("mr_k_xbd_ct");
+        } else {
+            mr_xbd_fn fn = masked ? mr_xbd_pick<true>(L) : mr_xbd_pick<false>(L);
+            hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(MR_XBD_THREADS), p->xbd_lds, s, B,
+                               ker, p->d_inv, kstride, d, p->fx, p->nxdiv, R, G, nc, ncg);
+            MR_LAUNCH_CHECK("mr_k_xbd");
+        }
         MR_TRY(mr_prof_mark(p, MR_ST_BD));
     }
     if (!two) {

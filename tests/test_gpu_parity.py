@@ -204,6 +204,53 @@ def test_fused_and_separate_routes(mr, monkeypatch, name, masked):
             assert rel(y, base) < 1e-6, (k, tri)
 
 
+SPECIALISED = [("2D", (320, 12), 16, 2), ("2D", (160, 10), 8, 3), ("2D", (128, 8), 4, 5),
+               ("2D", (192, 6), 16, 2), ("2D", (256, 6), 8, 3), ("stack", (320, 8, 3), 16, 2),
+               ("3D", (160, 6, 4), 4, 3), ("2D", (200, 6), 8, 2)]
+
+
+@pytest.mark.parametrize("tt,n,L,C", SPECIALISED)
+@pytest.mark.parametrize("masked", [False, True])
+def test_fused_specialised_lengths(mr, monkeypatch, tt, n, L, C, masked):
+    """x grid lengths 640, 320, 256, 384 and 512 with ranks 16, 8 and 4 run
+    the fused kernels compiled for their length; 400 (the last case) has no
+    such kernel and runs the run-time-length one. Random packed kernel and
+    data, both triangles, against the FP64 apply, the run-time-length fused
+    kernel and the separate passes."""
+    from oracle import conventions as cv
+    from oracle import toeplitz
+    print(f"[specialised] {tt} {n} L={L} C={C} masked={masked}", flush=True)
+    rng = np.random.default_rng(7 * L + C + len(n))
+    g = cv.make_geometry(tt, n, L, C)
+    T = g.T
+    grid = g.grid_shape()
+    mask = None
+    if masked:
+        keep = rng.random(g.P) < 0.6
+        mask = np.flatnonzero(keep).astype(np.int64)
+        kshape = (T, mask.size)
+    else:
+        kshape = (T,) + grid
+    K = (rng.standard_normal(kshape) + 1j * rng.standard_normal(kshape)).astype(np.complex64)
+    x = (rng.standard_normal((L,) + g.img_shape()) + 1j * rng.standard_normal((L,) + g.img_shape())).astype(np.complex64)
+    S = (rng.standard_normal((C,) + g.img_shape()) + 1j * rng.standard_normal((C,) + g.img_shape())).astype(np.complex64)
+    dx, dS, dK = dev(x), dev(S), dev(K)
+    dmask = dev(mask) if masked else None
+    for tri in ("lower", "upper"):
+        cfg = cfg_for(mr, g, triangulation=tri, is_masked=masked,
+                      num_masked=int(mask.size) if masked else 0)
+        ref = toeplitz.apply(g, x, S, K.astype(np.complex128), upper=(tri == "upper"), mask=mask)
+        outs = []
+        for xbd, ct in (("1", "1"), ("1", "0"), ("0", "1")):
+            monkeypatch.setenv("MR_AHA_HIP_XBD", xbd)
+            monkeypatch.setenv("MR_AHA_HIP_XBD_CT", ct)
+            assert (_xbd_group(mr, cfg) > 0) == (xbd == "1")
+            y = host(mr.aha_toeplitz(cfg, dx, dS, dK, dmask))
+            assert rel(y, ref) < 1e-5, (tri, xbd, ct, rel(y, ref))
+            outs.append(y)
+        assert rel(outs[0], outs[2]) < 1e-6 and rel(outs[1], outs[2]) < 1e-6
+
+
 @pytest.mark.parametrize("L", [1, 2, 5, 7, 12, 13, 16, 17, 20])
 def test_block_ranks(mr, L):
     """Every rank (specialised kernels up to 16, the any-rank kernel above)
