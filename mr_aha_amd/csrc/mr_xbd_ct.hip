@@ -89,9 +89,12 @@ constexpr int mr_xbd_ct_gmax(int N, int L)
 // One Stockham stage (radix R, Ns = NS) in place over tp pencils of length
 // N stored back to back in LDS. Phase 1 reads every butterfly input and
 // twiddle of the thread into registers, phase 2 (after the barrier) writes
-// the outputs.
+// the outputs. Rounds are skipped per wavefront (a scalar test on the
+// wavefront's first butterfly); inside a round, lanes past the last
+// butterfly repeat it, reading the same inputs and writing the same values
+// to the same places, so no per-lane mask has to live across the barrier.
 template <int N, int R, int NS, bool INV, int TPMAX>
-__device__ __forceinline__ void mr_ct_stage(float2 *A, __amdgpu_buffer_rsrc_t tw, int tp)
+__device__ __forceinline__ void mr_ct_stage(float2 *A, __amdgpu_buffer_rsrc_t tw, int tp, int wave)
 {
     constexpr int m = N / R;
     constexpr int NT = MR_XBD_THREADS;
@@ -103,8 +106,8 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, __amdgpu_buffer_rsrc_t tw
     float2 w[KMAX][RW];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
-        const unsigned f = threadIdx.x + k * NT;
-        if ((int)f < nb) {
+        if (k * NT + wave * 64 < nb) {
+            const unsigned f = min((unsigned)(threadIdx.x + k * NT), (unsigned)(nb - 1));
             const unsigned p = mr_cdiv<m>::q(f), j = f - p * m;
             const float2 *a = A + p * N + j;
 #pragma unroll
@@ -120,8 +123,8 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, __amdgpu_buffer_rsrc_t tw
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
-        const unsigned f = threadIdx.x + k * NT;
-        if ((int)f < nb) {
+        if (k * NT + wave * 64 < nb) {
+            const unsigned f = min((unsigned)(threadIdx.x + k * NT), (unsigned)(nb - 1));
             const unsigned p = mr_cdiv<m>::q(f), j = f - p * m;
             const unsigned jn = mr_cdiv<NS>::q(j), kk = j - jn * NS;
             if constexpr (NS > 1) {
@@ -139,13 +142,13 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, __amdgpu_buffer_rsrc_t tw
 }
 
 template <int N, int S, bool INV, int TPMAX>
-__device__ __forceinline__ void mr_ct_fft(float2 *A, __amdgpu_buffer_rsrc_t tw, int tp)
+__device__ __forceinline__ void mr_ct_fft(float2 *A, __amdgpu_buffer_rsrc_t tw, int tp, int wave)
 {
     constexpr mr_ct_fac F = mr_ct_factor(N);
     static_assert(F.rest == 1, "length must be 2^a 3^b 5^c 7^d");
     if constexpr (S < F.nst) {
-        mr_ct_stage<N, F.r[S], F.ns[S], INV, TPMAX>(A, tw, tp);
-        mr_ct_fft<N, S + 1, INV, TPMAX>(A, tw, tp);
+        mr_ct_stage<N, F.r[S], F.ns[S], INV, TPMAX>(A, tw, tp, wave);
+        mr_ct_fft<N, S + 1, INV, TPMAX>(A, tw, tp, wave);
     }
 }
 
@@ -187,7 +190,7 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
         }
     }
     __syncthreads();
-    mr_ct_fft<N, 0, false, TPMAX>(A, tw, tp);
+    mr_ct_fft<N, 0, false, TPMAX>(A, tw, tp, wave);
 
     const int My = d.m[1];
     const int jy = (int)(r % My), jz = (int)(r / My);
@@ -215,7 +218,7 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
         }
     }
     __syncthreads();
-    mr_ct_fft<N, 0, true, TPMAX>(A, tw, tp);
+    mr_ct_fft<N, 0, true, TPMAX>(A, tw, tp, wave);
 
     for (int pl = wave; pl < tp; pl += NW) {
         const __amdgpu_buffer_rsrc_t rs = mr_rsrc(Yr + pstride * pl, rowbytes);
