@@ -92,6 +92,31 @@ def apply_model_bytes(cfg):
     return T * P * 8 + 4 * C * L * P * 8 + (2 * L + 2 * C) * N * 8
 
 
+def calibrate_copy(nbytes=1 << 30, reps=4):
+    """Plain device copy of a known size, the reference point for scaling the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE readings of the library's kernels."""
+    import torch
+    x = torch.empty(nbytes // 8, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": reps}),
+          flush=True)
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 record (profiles/hbm_traffic.json), or None."""
+    path = ROOT / "profiles" / "hbm_traffic.json"
+    if not path.exists():
+        return None
+    for rec in json.loads(path.read_text()).get("records", []):
+        if rec.get("kernel") == kernel and rec.get("workload") == workload:
+            return int(rec["hbm_bytes_per_launch"])
+    return None
+
+
 def cpu_baseline(w, K, cfg, budget_s=30.0):
     """Oracle CPU Toeplitz apply on one coil of the workload (complex64)."""
     import numpy as np
@@ -120,7 +145,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="stack", choices=["stack", "radial", "koosh"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
         print("bench.py measures one GPU: the multi-GPU coil split is not part of this build "
@@ -194,6 +224,7 @@ def main():
             stages[name] = {"avg_ms": round(avg, 4), "launches": int(calls[i]),
                             "gbs": round(nb / (avg * 1e6), 1) if nb else None}
     dom = max((s for s in stages if s in sb), key=lambda s: stages[s]["avg_ms"])
+    dom_kernel = ({"BD": "mr_k_xbd_ct" if xbd else "mr_k_bd"}.get(dom, "mr_fft_axis_fast"))
     dom_ms = stages[dom]["avg_ms"]
     achieved = sb[dom] / (dom_ms * 1e6)
     B = apply_model_bytes(cfg)
@@ -215,9 +246,11 @@ def main():
                    "pipeline": (f"fused x pass, {xbd} coil(s) per workgroup" if xbd
                                 else "separate FFT and block passes"),
                    "kernel_build_s": round(info["toep_ms"] / 1e3, 2)},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "algorithmic_bytes_per_launch": sb[dom]},
+        "roofline": {"bound": "hbm", "kernel": dom_kernel, "stage": dom,
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(dom_kernel, workload),
+                     "algorithmic_bytes_per_launch": sb[dom]},
         "apply_model": {"bytes": B, "gbs": round(B / (ms_per_step * 1e6), 1),
                         "frac": round(B / (ms_per_step * 1e6) / HBM_PEAK_GBS, 4)},
         "stages": stages,
