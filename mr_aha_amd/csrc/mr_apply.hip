@@ -287,8 +287,8 @@ __device__ __forceinline__ void mr_bd_walk(const float2 (&v)[NC][L], float2 (&y)
         const float2 a = *bp;                                                  \
         bp += stride;                                                          \
         _Pragma("unroll") for (int k = 0; k < NC; ++k) {                       \
-            y[k][c] = cadd(y[k][c], cmul(a, v[k][r]));                         \
-            if ((r) != (c)) y[k][r] = cadd(y[k][r], cmulc(a, v[k][c]));        \
+            y[k][c] = cmac(y[k][c], a, v[k][r]);                         \
+            if ((r) != (c)) y[k][r] = cmacc(y[k][r], a, v[k][c]);        \
         }                                                                      \
     } while (0)
     if (upper) {
@@ -395,7 +395,7 @@ mr_k_bd_any(float2 *__restrict__ K, const float2 *__restrict__ ker, const int *_
                 else { if (r < cc) { int t = r; r = cc; cc = t; cj = 1; } }
                 const int t = d.tri_upper ? cc * (cc + 1) / 2 + r : cc * L - cc * (cc - 1) / 2 + (r - cc);
                 const float2 a = base[kstride * t];
-                acc = cadd(acc, cj ? cmulc(a, v[j]) : cmul(a, v[j]));
+                acc = cj ? cmacc(acc, a, v[j]) : cmac(acc, a, v[j]);
             }
             g[P * i] = acc;
         }

@@ -45,6 +45,25 @@ template <typename V> __device__ __forceinline__ V cmulc(V a, V b)
     return cmake<V>(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 template <typename V> __device__ __forceinline__ V cconj(V a) { return cmake<V>(a.x, -a.y); }
+// y + a * b and y + conj(a) * b as four fused multiply-adds (a complex
+// product followed by an add would take six instructions: the compiler may
+// not reassociate the add into the product)
+__device__ __forceinline__ float2 cmac(float2 y, float2 a, float2 b)
+{
+    y.x = __builtin_fmaf(a.x, b.x, y.x);
+    y.x = __builtin_fmaf(-a.y, b.y, y.x);
+    y.y = __builtin_fmaf(a.x, b.y, y.y);
+    y.y = __builtin_fmaf(a.y, b.x, y.y);
+    return y;
+}
+__device__ __forceinline__ float2 cmacc(float2 y, float2 a, float2 b)
+{
+    y.x = __builtin_fmaf(a.x, b.x, y.x);
+    y.x = __builtin_fmaf(a.y, b.y, y.x);
+    y.y = __builtin_fmaf(a.x, b.y, y.y);
+    y.y = __builtin_fmaf(-a.y, b.x, y.y);
+    return y;
+}
 // multiply by -i (forward) or +i (inverse)
 template <bool INV, typename V> __device__ __forceinline__ V cmul_mi(V a)
 {

@@ -93,8 +93,8 @@ __device__ __forceinline__ void mr_bd_walk_g(float2 *a, int ms, const float2 *ke
                 const int r = r0 + j;
                 const float2 e = kk[i & 1][j];
                 const float2 vr = (r == c) ? vc : a[r * ms];
-                y[c] = cadd(y[c], cmul(e, vr));
-                if (r != c) y[r] = cadd(y[r], cmulc(e, vc));
+                y[c] = cmac(y[c], e, vr);
+                if (r != c) y[r] = cmacc(y[r], e, vc);
             }
         }
         a[c * ms] = y[c];
