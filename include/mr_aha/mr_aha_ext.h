@@ -35,6 +35,24 @@ MR_EXPORT int mr_op_profile(mr_op op, double *ms, int *calls, int n);
 MR_EXPORT int mr_aha_plan_xbd_group(mr_aha_plan plan);
 MR_EXPORT int mr_op_xbd_group(mr_op op);
 
+/* Conjugate gradients on (A^H A + lambda I) x = b with operator `op`
+ * (FP32 vectors; FP64 inner products and step lengths, kept on the device).
+ * Every `refresh` iterations (0 = never) the recursive residual is replaced
+ * by the true residual b - (A^H A + lambda I) x, computed with `op_refresh`
+ * (same shape, device and stream; e.g. the FP32 operator when `op` holds a
+ * 16-bit kernel) or with `op` when op_refresh is NULL. d_x holds the initial
+ * guess (or is zeroed with MR_CG_ZERO_INIT) and receives the solution.
+ * res_hist (host, optional) receives ||r_k|| / ||b|| for k = 0..n_iter, r_k
+ * being the residual the iteration carries. MR_CG_GRAPH captures one
+ * ordinary and one refresh iteration as HIP graphs and replays them (needs a
+ * non-default cfg.stream). Synchronises with the stream before returning.
+ * No counterpart in the reference: its callers run CG on top of
+ * mr_op_execute (reference README.md:70-73). */
+#define MR_CG_ZERO_INIT 1
+#define MR_CG_GRAPH 2
+MR_EXPORT int mr_op_cg(mr_op op, mr_op op_refresh, const mr_complex *d_b, mr_complex *d_x,
+                       double lambda, int n_iter, int refresh, int flags, double *res_hist);
+
 #ifdef __cplusplus
 }
 #endif

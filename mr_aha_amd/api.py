@@ -26,8 +26,8 @@ from typing import Optional, Sequence
 
 import torch
 
-from ._lib import (DEFAULT_MASK_RADIUS, OP_ALL_DEVICES, OP_SHARE_PRIMARY, OP_TOEP_SOURCES,
-                   MRConfig, MRError, MROpInfo, MROpInputs, MROpOptions, OpMethod, ToepDist,
+from ._lib import (CG_GRAPH, CG_ZERO_INIT, DEFAULT_MASK_RADIUS, OP_ALL_DEVICES,
+                   OP_SHARE_PRIMARY, OP_TOEP_SOURCES, MRConfig, MRError, MROpInfo, MROpInputs, MROpOptions, OpMethod, ToepDist,
                    TrajType, Triangulation, check, lib)
 
 __all__ = ["DEFAULT_MASK_RADIUS", "MRConfig", "MRError", "TrajType", "Triangulation",
@@ -366,6 +366,37 @@ class AhAOp:
         nbytes = out.numel() * 8
         _memcpy_d2d(out.data_ptr(), ptr.value, nbytes)
         return out
+
+    def cg(self, b, x=None, *, lam: float = 0.0, iters: int = 40, refresh: int = 10,
+           op_refresh: Optional["AhAOp"] = None, graph: bool = False):
+        """Conjugate gradients on (A^H A + lam I) x = b (library entry mr_op_cg).
+
+        FP32 vectors, FP64 inner products and step lengths; every ``refresh``
+        iterations the residual is recomputed as b - (A^H A + lam I) x with
+        ``op_refresh`` (default: this operator). ``x`` is the initial guess
+        (zeros when None) and is overwritten. ``graph`` replays HIP graphs of
+        the iterations (needs a configuration built with a non-default stream).
+        Returns ``(x, hist)`` with hist[k] = ||r_k|| / ||b||, k = 0..iters.
+        No counterpart in the reference, whose callers run CG in Python.
+        """
+        self._check_open()
+        n = int(lib.mr_x_size(ctypes.byref(self._cfg)))
+        _check_size(b, n, "b")
+        flags = 0
+        if x is None:
+            x = torch.empty(b.shape, dtype=torch.complex64, device=b.device)
+            flags |= CG_ZERO_INIT
+        else:
+            _check_size(x, n, "x")
+        if graph:
+            flags |= CG_GRAPH
+        hist = (ctypes.c_double * (int(iters) + 1))()
+        ref = op_refresh._handle if op_refresh is not None else None
+        if op_refresh is not None:
+            op_refresh._check_open()
+        check(lib.mr_op_cg(self._handle, ref, _ptr(b, "b", "complex64"), _ptr(x, "x", "complex64"),
+                           float(lam), int(iters), int(refresh), flags, hist), "mr_op_cg")
+        return x, list(hist)
 
     def close(self) -> None:
         if self._handle:

@@ -363,6 +363,16 @@ extern "C" MR_EXPORT int mr_op_profile(mr_op op, double *ms, int *calls, int n)
     return mr_aha_plan_profile(op->plan, ms, calls, n);
 }
 
+// Stream, image-vector length (N * L) and device of an operator.
+int mr_op_stream_len(mr_op op, hipStream_t *s, long long *n, int *device)
+{
+    MR_REQUIRE(op != nullptr, "operator is NULL");
+    *s = op->stream;
+    *n = (long long)mr_x_size(&op->cfg);
+    *device = op->primary;
+    return MR_SUCCESS;
+}
+
 extern "C" MR_EXPORT int mr_op_xbd_group(mr_op op)
 {
     mr_clear_error();
