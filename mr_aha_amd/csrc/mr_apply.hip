@@ -23,6 +23,7 @@
 #include "mr_hip.hpp"
 #include "mr_aha/mr_aha_ext.h"
 #include "mr_xbd.hpp"
+#include "mr_fft_ct.hpp"
 
 #include <cstdlib>
 #include <cstring>
@@ -928,13 +929,13 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
         mr_zmap zm{V, lgT};
         mr_ld_sx_z ld{x, S, zm, plane, N, No, Mo, cno, L, coil0};
         mr_st_z st{A, zm, plane, Mo};
-        MR_TRY((mr_fft_pass<float, false>(fo, ld, st, plane * V, 0, 1 << lgT, s, "Fo")));
+        MR_TRY((mr_fft_pass_ct<false>(fo, ld, st, plane * V, 0, 1 << lgT, s, "Fo")));
         MR_TRY(mr_prof_mark(p, two ? MR_ST_F2 : MR_ST_F3));
     }
     if (!two) {
         mr_ld_pad<float2, false> ld{A, Nx, My, Ny, d.c_n[1]};
         mr_st_axis<float2, false> st{B, Nx, My};
-        MR_TRY((mr_fft_pass<float, false>(p->fy, ld, st, (long long)Nx * Mz * V, 0, 16, s, "Fy")));
+        MR_TRY((mr_fft_pass_ct<false>(p->fy, ld, st, (long long)Nx * Mz * V, 0, 16, s, "Fy")));
         MR_TRY(mr_prof_mark(p, MR_ST_F2));
     }
     {
@@ -962,14 +963,14 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
     if (!two) {
         mr_ld_axis<float2, false> ld{B, Nx, My};
         mr_st_crop<float2, false> st{A, Nx, My, Ny, d.c_n[1]};
-        MR_TRY((mr_fft_pass<float, true>(p->fy, ld, st, (long long)Nx * Mz * V, 0, 16, s, "Iy")));
+        MR_TRY((mr_fft_pass_ct<true>(p->fy, ld, st, (long long)Nx * Mz * V, 0, 16, s, "Iy")));
         MR_TRY(mr_prof_mark(p, MR_ST_I2));
     }
     {
         mr_zmap zm{L, lgT};
         mr_ld_z_coil ld{A, zm, plane, Mo, L, nc};
         mr_st_z_conj_sum st{out, S, zm, plane, N, No, Mo, cno, coil0, coil0 > 0 ? 1 : 0};
-        MR_TRY((mr_fft_pass<float, true>(fo, ld, st, plane * L, (size_t)No, 1 << lgT, s, "Io")));
+        MR_TRY((mr_fft_pass_ct<true>(fo, ld, st, plane * L, (size_t)No, 1 << lgT, s, "Io")));
         MR_TRY(mr_prof_mark(p, two ? MR_ST_I2 : MR_ST_I3));
     }
     return MR_SUCCESS;

@@ -1,0 +1,258 @@
+// Compile-time FFT stages: the length N, its radices and every stride and
+// divisor are constants, so divisions become one full-rate 24-bit multiply
+// and a shift, LDS offsets become instruction immediates, and no run-time
+// plan occupies scalar registers. Same self-sorting Stockham algorithm and
+// radix rule as the run-time engine (mr_fft.hpp), and the same stage-major
+// twiddle table (entry r*Ns + k - 1 of stage (R, Ns)).
+#pragma once
+
+#include "mr_hip.hpp"
+
+// Stage radices of n by the rule of mr_fft_factor (8s, then 4, 2, 3, 5, 7)
+// and the product Ns of the radices before each stage.
+struct mr_ct_fac {
+    int nst;
+    int r[16];
+    int ns[16];
+    int rest;  // 1 when n is smooth
+};
+
+constexpr mr_ct_fac mr_ct_factor(int n)
+{
+    mr_ct_fac f{};
+    const int small[6] = {8, 4, 2, 3, 5, 7};
+    int m = n, Ns = 1;
+    for (int i = 0; i < 6; ++i)
+        while (m % small[i] == 0 && f.nst < 16) {
+            f.r[f.nst] = small[i];
+            f.ns[f.nst] = Ns;
+            Ns *= small[i];
+            ++f.nst;
+            m /= small[i];
+        }
+    f.rest = m;
+    return f;
+}
+
+constexpr int mr_ct_clog2(int d)
+{
+    int k = 0;
+    while ((1 << k) < d) ++k;
+    return k;
+}
+
+// x / D for 0 <= x < 2^14 as (x * M) >> K with M = ceil(2^K / D),
+// K = 14 + ceil(log2 D): the rounding error x (M D - 2^K) / (D 2^K) stays
+// below 2^-ceil(log2 D) <= 1/D, so the floor is exact; x * M < 2^30 fits
+// one 24-bit multiply. Powers of two shift.
+template <int D> struct mr_cdiv {
+    static_assert(D >= 1 && D <= (1 << 14), "divisor range");
+    static constexpr bool POW2 = (D & (D - 1)) == 0;
+    static constexpr int K = 14 + mr_ct_clog2(D);
+    static constexpr unsigned M = POW2 ? 0u : (unsigned)(((1ull << K) + D - 1) / D);
+    __device__ static __forceinline__ unsigned q(unsigned x)
+    {
+        if constexpr (POW2) return x >> mr_ct_clog2(D);
+        else return __umul24(x, M) >> K;
+    }
+};
+
+// Exhaustive compile-time check of mr_cdiv over its whole domain.
+template <int D> constexpr bool mr_cdiv_exact()
+{
+    if (mr_cdiv<D>::POW2) return true;
+    for (unsigned x = 0; x < (1u << 14); ++x)
+        if ((unsigned)(((unsigned long long)x * mr_cdiv<D>::M) >> mr_cdiv<D>::K) != x / D)
+            return false;
+    return true;
+}
+
+// Twiddle sources: a buffer resource over the global table, or the table
+// copied into LDS.
+struct mr_tw_buf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ float2 get(unsigned idx, unsigned off) const
+    {
+        const unsigned int __attribute__((ext_vector_type(2))) v =
+            __builtin_amdgcn_raw_buffer_load_b64(r, (int)(idx * 8u), (int)(off * 8u), 0);
+        return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+    }
+};
+struct mr_tw_lds {
+    const float2 *t;
+    __device__ __forceinline__ float2 get(unsigned idx, unsigned off) const { return t[idx + off]; }
+};
+
+// One Stockham stage (radix R, Ns = NS) in place over tp pencils of length N
+// at pencil stride NSTR in LDS, NT threads. Phase 1 reads every butterfly
+// input and twiddle of the thread into registers, phase 2 (after the
+// barrier) writes the outputs. Rounds are skipped per wavefront (a scalar
+// test on the wavefront's first butterfly); inside a round, lanes past the
+// last butterfly repeat it, reading the same inputs and writing the same
+// values to the same places, so no per-lane mask lives across the barrier.
+template <int N, int NSTR, int R, int NS, bool INV, int NT, int TPMAX, class TW>
+__device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int wave)
+{
+    constexpr int m = N / R;
+    constexpr int KMAX = (TPMAX * m + NT - 1) / NT;
+    constexpr int RW = (R > 1) ? R - 1 : 1;
+    static_assert(mr_cdiv_exact<m>() && mr_cdiv_exact<NS>(), "inexact constant division");
+    const int nb = tp * m;
+    float2 v[KMAX][R];
+    float2 w[KMAX][RW];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (k * NT + wave * 64 < nb) {
+            const unsigned f = min((unsigned)(threadIdx.x + k * NT), (unsigned)(nb - 1));
+            const unsigned p = mr_cdiv<m>::q(f), j = f - p * m;
+            const float2 *a = A + p * NSTR + j;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[k][r] = a[r * m];
+            if constexpr (NS > 1) {
+                const unsigned kk = j - mr_cdiv<NS>::q(j) * NS;
+#pragma unroll
+                for (int r = 1; r < R; ++r) w[k][r - 1] = tw.get(NS + kk - 1, (unsigned)((r - 1) * NS));
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (k * NT + wave * 64 < nb) {
+            const unsigned f = min((unsigned)(threadIdx.x + k * NT), (unsigned)(nb - 1));
+            const unsigned p = mr_cdiv<m>::q(f), j = f - p * m;
+            const unsigned jn = mr_cdiv<NS>::q(j), kk = j - jn * NS;
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    v[k][r] = INV ? cmulc(w[k][r - 1], v[k][r]) : cmul(w[k][r - 1], v[k][r]);
+            }
+            bfly<R, INV>(v[k]);
+            float2 *b = A + p * NSTR + jn * (NS * R) + kk;
+#pragma unroll
+            for (int q = 0; q < R; ++q) b[q * NS] = v[k][q];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int NSTR, int S, bool INV, int NT, int TPMAX, class TW>
+__device__ __forceinline__ void mr_ct_fft(float2 *A, const TW &tw, int tp, int wave)
+{
+    constexpr mr_ct_fac F = mr_ct_factor(N);
+    static_assert(F.rest == 1, "length must be 2^a 3^b 5^c 7^d");
+    if constexpr (S < F.nst) {
+        mr_ct_stage<N, NSTR, F.r[S], F.ns[S], INV, NT, TPMAX>(A, tw, tp, wave);
+        mr_ct_fft<N, NSTR, S + 1, INV, NT, TPMAX>(A, tw, tp, wave);
+    }
+}
+
+// Axis pass of the FFT engine at compile-time length N (same functor
+// protocol and tiling as mr_fft_axis_fast): TX = 2^lgTX <= TPMAX pencils per
+// workgroup, 512 threads, twiddles in LDS.
+template <int N> constexpr int mr_ct_axis_tpmax()
+{
+    int t = 64;
+    while (t > 1 && t * N > MR_FFT_EMAX) t /= 2;
+    return t;
+}
+
+template <int N, bool INV, class Load, class Store>
+__global__ void __launch_bounds__(MR_FFT_FAST_THREADS)
+mr_fft_axis_ct(const float2 *tws, Load ld, Store st, long long npencil, int lgTX)
+{
+    constexpr int NT = MR_FFT_FAST_THREADS;
+    constexpr int NSTR = (N % 2 == 0) ? N + 1 : N;
+    constexpr int TPMAX = mr_ct_axis_tpmax<N>();
+    extern __shared__ __align__(16) unsigned char mr_smem[];
+    __shared__ typename Load::PB lpb[MR_FFT_MAXTX];
+    __shared__ typename Store::PB spb[MR_FFT_MAXTX];
+    const int TX = 1 << lgTX;
+    float2 *twl = reinterpret_cast<float2 *>(mr_smem);
+    float2 *A = twl + N;
+    float2 *acc = A + (size_t)TX * NSTR;
+    const long long p0 = mr_xcd_block(blockIdx.x, gridDim.x) * TX;
+    const int tp = (int)((npencil - p0 < TX) ? (npencil - p0) : TX);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int i = threadIdx.x; i < N; i += NT) twl[i] = tws[i];
+    const mr_tw_lds tw{twl};
+    const int nsub = ld.nsub();
+    for (int sub = 0; sub < nsub; ++sub) {
+        if (threadIdx.x < tp) {
+            lpb[threadIdx.x] = ld.pencil(p0 + threadIdx.x, sub);
+            spb[threadIdx.x] = st.pencil(p0 + threadIdx.x, sub);
+        }
+        __syncthreads();
+        if (Load::kElemFast) {
+            for (int q = threadIdx.x; q < tp * N; q += NT) {
+                const int p = (int)mr_cdiv<N>::q((unsigned)q), e = q - p * N;
+                A[p * NSTR + e] = ld.get(lpb[p], e);
+            }
+        } else {
+            for (int q = threadIdx.x; q < (N << lgTX); q += NT) {
+                const int e = q >> lgTX, p = q & (TX - 1);
+                if (p < tp) A[p * NSTR + e] = ld.get(lpb[p], e);
+            }
+        }
+        __syncthreads();
+        mr_ct_fft<N, NSTR, 0, INV, NT, TPMAX>(A, tw, tp, wave);
+        if (Store::kElemFast) {
+            for (int q = threadIdx.x; q < tp * N; q += NT) {
+                const int p = (int)mr_cdiv<N>::q((unsigned)q), e = q - p * N;
+                st.put(spb[p], p, e, sub, A[p * NSTR + e], acc);
+            }
+        } else {
+            for (int q = threadIdx.x; q < (N << lgTX); q += NT) {
+                const int e = q >> lgTX, p = q & (TX - 1);
+                if (p < tp) st.put(spb[p], p, e, sub, A[p * NSTR + e], acc);
+            }
+        }
+        __syncthreads();
+    }
+    st.flush(p0, tp, acc);
+}
+
+// Host: launch the compile-time kernel for length N (tile choice as in
+// mr_fft_pass).
+template <int N, bool INV, class Load, class Store>
+int mr_fft_pass_ct_n(const mr_fft_len<float> &f, const Load &ld, const Store &st,
+                     long long npencil, size_t acc_per_pencil, int want_tx, hipStream_t stream,
+                     const char *name)
+{
+    constexpr int NSTR = (N % 2 == 0) ? N + 1 : N;
+    if (npencil <= 0) return MR_SUCCESS;
+    int TX = want_tx > MR_FFT_MAXTX ? MR_FFT_MAXTX : want_tx;
+    auto need = [&](int tx) {
+        return ((size_t)N + (size_t)tx * NSTR + (size_t)tx * acc_per_pencil) * sizeof(float2);
+    };
+    while (TX > 1 && ((long long)TX * N > MR_FFT_EMAX || need(TX) > 64 * 1024)) TX /= 2;
+    while (TX > 1 && TX / 2 >= npencil) TX /= 2;
+    int lg = 0;
+    while ((1 << lg) < TX) ++lg;
+    const long long blocks = (npencil + TX - 1) / TX;
+    if (blocks > 0x7fffffffLL)
+        MR_FAIL(MR_ERR_UNSUPPORTED, "%s: %lld pencils exceed the launch limit", name, npencil);
+    hipLaunchKernelGGL((mr_fft_axis_ct<N, INV, Load, Store>), dim3((unsigned)blocks),
+                       dim3(MR_FFT_FAST_THREADS), need(TX), stream, f.tws, ld, st, npencil, lg);
+    MR_LAUNCH_CHECK(name);
+    return MR_SUCCESS;
+}
+
+// FP32 axis pass: the compile-time kernel for lengths 56, 256, 320, 384, 512
+// and 640 (MR_AHA_HIP_FFT_CT=0 turns it off), mr_fft_pass otherwise.
+template <bool INV, class Load, class Store>
+int mr_fft_pass_ct(const mr_fft_len<float> &f, const Load &ld, const Store &st, long long npencil,
+                   size_t acc_per_pencil, int want_tx, hipStream_t stream, const char *name)
+{
+    if (mr_env_ll("MR_AHA_HIP_FFT_CT", 1) != 0) {
+        switch (f.n) {
+#define MR_CT_LEN(n)                                                                             \
+    case n:                                                                                      \
+        return mr_fft_pass_ct_n<n, INV>(f, ld, st, npencil, acc_per_pencil, want_tx, stream, name);
+            MR_CT_LEN(56) MR_CT_LEN(256) MR_CT_LEN(320) MR_CT_LEN(384) MR_CT_LEN(512) MR_CT_LEN(640)
+#undef MR_CT_LEN
+        default: break;
+        }
+    }
+    return mr_fft_pass<float, INV>(f, ld, st, npencil, acc_per_pencil, want_tx, stream, name);
+}

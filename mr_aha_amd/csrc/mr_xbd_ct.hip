@@ -14,67 +14,9 @@
 // other shapes run mr_k_xbd.
 
 #include "mr_xbd.hpp"
+#include "mr_fft_ct.hpp"
 
 namespace {
-
-// Stage radices of n by the rule of mr_fft_factor (8s, then 4, 2, 3, 5, 7)
-// and the product Ns of the radices before each stage.
-struct mr_ct_fac {
-    int nst;
-    int r[16];
-    int ns[16];
-    int rest;  // 1 when n is smooth
-};
-
-constexpr mr_ct_fac mr_ct_factor(int n)
-{
-    mr_ct_fac f{};
-    const int small[6] = {8, 4, 2, 3, 5, 7};
-    int m = n, Ns = 1;
-    for (int i = 0; i < 6; ++i)
-        while (m % small[i] == 0 && f.nst < 16) {
-            f.r[f.nst] = small[i];
-            f.ns[f.nst] = Ns;
-            Ns *= small[i];
-            ++f.nst;
-            m /= small[i];
-        }
-    f.rest = m;
-    return f;
-}
-
-constexpr int mr_ct_clog2(int d)
-{
-    int k = 0;
-    while ((1 << k) < d) ++k;
-    return k;
-}
-
-// x / D for 0 <= x < 2^14 as (x * M) >> K with M = ceil(2^K / D),
-// K = 14 + ceil(log2 D): the rounding error x (M D - 2^K) / (D 2^K) stays
-// below 2^-ceil(log2 D) <= 1/D, so the floor is exact; x * M < 2^30 fits
-// one 24-bit multiply. Powers of two shift.
-template <int D> struct mr_cdiv {
-    static_assert(D >= 1 && D <= (1 << 14), "divisor range");
-    static constexpr bool POW2 = (D & (D - 1)) == 0;
-    static constexpr int K = 14 + mr_ct_clog2(D);
-    static constexpr unsigned M = POW2 ? 0u : (unsigned)(((1ull << K) + D - 1) / D);
-    __device__ static __forceinline__ unsigned q(unsigned x)
-    {
-        if constexpr (POW2) return x >> mr_ct_clog2(D);
-        else return __umul24(x, M) >> K;
-    }
-};
-
-// Exhaustive compile-time check of mr_cdiv over its whole domain.
-template <int D> constexpr bool mr_cdiv_exact()
-{
-    if (mr_cdiv<D>::POW2) return true;
-    for (unsigned x = 0; x < (1u << 14); ++x)
-        if ((unsigned)(((unsigned long long)x * mr_cdiv<D>::M) >> mr_cdiv<D>::K) != x / D)
-            return false;
-    return true;
-}
 
 // Coils one workgroup may take: LDS within 80 KB (two workgroups per CU)
 // and G*L*N <= MR_XBD_EMAX.
@@ -84,72 +26,6 @@ constexpr int mr_xbd_ct_gmax(int N, int L)
     const int by_e = MR_XBD_EMAX / (L * N);
     const int g = by_lds < by_e ? by_lds : by_e;
     return g < 1 ? 1 : g;
-}
-
-// One Stockham stage (radix R, Ns = NS) in place over tp pencils of length
-// N stored back to back in LDS. Phase 1 reads every butterfly input and
-// twiddle of the thread into registers, phase 2 (after the barrier) writes
-// the outputs. Rounds are skipped per wavefront (a scalar test on the
-// wavefront's first butterfly); inside a round, lanes past the last
-// butterfly repeat it, reading the same inputs and writing the same values
-// to the same places, so no per-lane mask has to live across the barrier.
-template <int N, int R, int NS, bool INV, int TPMAX>
-__device__ __forceinline__ void mr_ct_stage(float2 *A, __amdgpu_buffer_rsrc_t tw, int tp, int wave)
-{
-    constexpr int m = N / R;
-    constexpr int NT = MR_XBD_THREADS;
-    constexpr int KMAX = (TPMAX * m + NT - 1) / NT;
-    constexpr int RW = (R > 1) ? R - 1 : 1;
-    static_assert(mr_cdiv_exact<m>() && mr_cdiv_exact<NS>(), "inexact constant division");
-    const int nb = tp * m;
-    float2 v[KMAX][R];
-    float2 w[KMAX][RW];
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-        if (k * NT + wave * 64 < nb) {
-            const unsigned f = min((unsigned)(threadIdx.x + k * NT), (unsigned)(nb - 1));
-            const unsigned p = mr_cdiv<m>::q(f), j = f - p * m;
-            const float2 *a = A + p * N + j;
-#pragma unroll
-            for (int r = 0; r < R; ++r) v[k][r] = a[r * m];
-            if constexpr (NS > 1) {
-                const unsigned kk = j - mr_cdiv<NS>::q(j) * NS;
-                const unsigned off = (NS + kk - 1) * 8u;
-#pragma unroll
-                for (int r = 1; r < R; ++r) w[k][r - 1] = mr_bload(tw, off, (unsigned)((r - 1) * NS * 8));
-            }
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-        if (k * NT + wave * 64 < nb) {
-            const unsigned f = min((unsigned)(threadIdx.x + k * NT), (unsigned)(nb - 1));
-            const unsigned p = mr_cdiv<m>::q(f), j = f - p * m;
-            const unsigned jn = mr_cdiv<NS>::q(j), kk = j - jn * NS;
-            if constexpr (NS > 1) {
-#pragma unroll
-                for (int r = 1; r < R; ++r)
-                    v[k][r] = INV ? cmulc(w[k][r - 1], v[k][r]) : cmul(w[k][r - 1], v[k][r]);
-            }
-            bfly<R, INV>(v[k]);
-            float2 *b = A + p * N + jn * (NS * R) + kk;
-#pragma unroll
-            for (int q = 0; q < R; ++q) b[q * NS] = v[k][q];
-        }
-    }
-    __syncthreads();
-}
-
-template <int N, int S, bool INV, int TPMAX>
-__device__ __forceinline__ void mr_ct_fft(float2 *A, __amdgpu_buffer_rsrc_t tw, int tp, int wave)
-{
-    constexpr mr_ct_fac F = mr_ct_factor(N);
-    static_assert(F.rest == 1, "length must be 2^a 3^b 5^c 7^d");
-    if constexpr (S < F.nst) {
-        mr_ct_stage<N, F.r[S], F.ns[S], INV, TPMAX>(A, tw, tp, wave);
-        mr_ct_fft<N, S + 1, INV, TPMAX>(A, tw, tp, wave);
-    }
 }
 
 template <int N, int L, bool MASKED>
@@ -169,7 +45,7 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
     const int tp = gn * L;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const __amdgpu_buffer_rsrc_t tw = mr_rsrc(tws, (unsigned)N * 8u);
+    const mr_tw_buf twb{mr_rsrc(tws, (unsigned)N * 8u)};
     // pencil pl = g*L + l of the workgroup is volume v = L*c0 + pl; every
     // wavefront loads and stores whole pencils, so a pencil's base is uniform
     const float2 *Yr = Y + (long long)Nx * r + (long long)Nx * R * L * c0;
@@ -190,7 +66,7 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
         }
     }
     __syncthreads();
-    mr_ct_fft<N, 0, false, TPMAX>(A, tw, tp, wave);
+    mr_ct_fft<N, N, 0, false, MR_XBD_THREADS, TPMAX>(A, twb, tp, wave);
 
     const int My = d.m[1];
     const int jy = (int)(r % My), jz = (int)(r / My);
@@ -218,7 +94,7 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
         }
     }
     __syncthreads();
-    mr_ct_fft<N, 0, true, TPMAX>(A, tw, tp, wave);
+    mr_ct_fft<N, N, 0, true, MR_XBD_THREADS, TPMAX>(A, twb, tp, wave);
 
     for (int pl = wave; pl < tp; pl += NW) {
         const __amdgpu_buffer_rsrc_t rs = mr_rsrc(Yr + pstride * pl, rowbytes);

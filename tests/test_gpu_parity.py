@@ -251,6 +251,36 @@ def test_fused_specialised_lengths(mr, monkeypatch, tt, n, L, C, masked):
         assert rel(outs[0], outs[2]) < 1e-6 and rel(outs[1], outs[2]) < 1e-6
 
 
+CT_AXIS = [("stack", (20, 128, 56), 4, 2), ("3D", (20, 16, 160), 4, 2), ("2D", (24, 160), 8, 3),
+           ("2D", (40, 192), 4, 2), ("stack", (18, 256, 8), 8, 2)]
+
+
+@pytest.mark.parametrize("tt,n,L,C", CT_AXIS)
+def test_fft_compile_time_lengths(mr, monkeypatch, tt, n, L, C):
+    """y and z grid lengths 256, 56, 320, 384 and 512 run the axis passes
+    compiled for their length (outer-axis pass with the sensitivity product,
+    y pass with padding, inverse y pass with crop, inverse outer pass with the
+    coil sum); against the FP64 apply and the run-time-length passes."""
+    from oracle import conventions as cv
+    from oracle import toeplitz
+    print(f"[ct-axis] {tt} {n} L={L} C={C}", flush=True)
+    rng = np.random.default_rng(11 * L + C)
+    g = cv.make_geometry(tt, n, L, C)
+    kshape = (g.T,) + g.grid_shape()
+    K = (rng.standard_normal(kshape) + 1j * rng.standard_normal(kshape)).astype(np.complex64)
+    x = (rng.standard_normal((L,) + g.img_shape()) + 1j * rng.standard_normal((L,) + g.img_shape())).astype(np.complex64)
+    S = (rng.standard_normal((C,) + g.img_shape()) + 1j * rng.standard_normal((C,) + g.img_shape())).astype(np.complex64)
+    cfg = cfg_for(mr, g)
+    ref = toeplitz.apply(g, x, S, K.astype(np.complex128))
+    outs = []
+    for ct in ("1", "0"):
+        monkeypatch.setenv("MR_AHA_HIP_FFT_CT", ct)
+        y = host(mr.aha_toeplitz(cfg, dev(x), dev(S), dev(K)))
+        assert rel(y, ref) < 1e-5, (ct, rel(y, ref))
+        outs.append(y)
+    assert rel(outs[0], outs[1]) < 1e-6
+
+
 @pytest.mark.parametrize("L", [1, 2, 5, 7, 12, 13, 16, 17, 20])
 def test_block_ranks(mr, L):
     """Every rank (specialised kernels up to 16, the any-rank kernel above)
