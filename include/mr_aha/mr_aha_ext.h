@@ -48,6 +48,22 @@ MR_EXPORT int mr_op_xbd_group(mr_op op);
  * non-default cfg.stream). Synchronises with the stream before returning.
  * No counterpart in the reference: its callers run CG on top of
  * mr_op_execute (reference README.md:70-73). */
+/* Kernel storage format of a Toeplitz operator's applies. MR_KERNEL_BF16 and
+ * MR_KERNEL_FP16 quantise the operator's FP32 kernel once (each Hermitian
+ * entry as two 16-bit values scaled by the largest component of its 4x4
+ * block of the L x L matrix, one FP32 scale per block and location) and the
+ * fused x pass reads that instead: half the kernel bytes per apply, at the
+ * precision of the format (relative entry error 2^-9 BF16, 2^-11 FP16). The
+ * FP32 kernel is kept: MR_KERNEL_FP32 switches back. Available where the
+ * fused x pass is compiled for the shape (x grid 256 with L = 4, 320 with
+ * L = 8, 640 with L = 16; unmasked); MR_ERR_UNSUPPORTED otherwise. Pair with
+ * mr_op_cg's op_refresh = an FP32 operator for iterative refinement. No
+ * counterpart in the reference. */
+#define MR_KERNEL_FP32 0
+#define MR_KERNEL_BF16 1
+#define MR_KERNEL_FP16 2
+MR_EXPORT int mr_op_set_kernel_format(mr_op op, int format);
+
 #define MR_CG_ZERO_INIT 1
 #define MR_CG_GRAPH 2
 MR_EXPORT int mr_op_cg(mr_op op, mr_op op_refresh, const mr_complex *d_b, mr_complex *d_x,

@@ -194,6 +194,8 @@ _decl("mr_op_cg", ctypes.c_int, [_p, _p, _p, _p, ctypes.c_double, ctypes.c_int, 
                                  ctypes.c_int, ctypes.POINTER(ctypes.c_double)])
 CG_ZERO_INIT = 1
 CG_GRAPH = 2
+_decl("mr_op_set_kernel_format", ctypes.c_int, [_p, ctypes.c_int])
+KERNEL_FP32, KERNEL_BF16, KERNEL_FP16 = 0, 1, 2
 PROFILE_STAGES = ("F1", "F2", "F3", "BD", "I3", "I2", "I1", "other")
 _decl("mr_calc_mask", ctypes.c_int, [_cfg, _p, ctypes.c_int64, ctypes.c_int64, _p,
                                      ctypes.POINTER(ctypes.c_int64)])

@@ -26,8 +26,8 @@ from typing import Optional, Sequence
 
 import torch
 
-from ._lib import (CG_GRAPH, CG_ZERO_INIT, DEFAULT_MASK_RADIUS, OP_ALL_DEVICES,
-                   OP_SHARE_PRIMARY, OP_TOEP_SOURCES, MRConfig, MRError, MROpInfo, MROpInputs, MROpOptions, OpMethod, ToepDist,
+from ._lib import (CG_GRAPH, CG_ZERO_INIT, DEFAULT_MASK_RADIUS, KERNEL_BF16, KERNEL_FP16,
+                   KERNEL_FP32, OP_ALL_DEVICES, OP_SHARE_PRIMARY, OP_TOEP_SOURCES, MRConfig, MRError, MROpInfo, MROpInputs, MROpOptions, OpMethod, ToepDist,
                    TrajType, Triangulation, check, lib)
 
 __all__ = ["DEFAULT_MASK_RADIUS", "MRConfig", "MRError", "TrajType", "Triangulation",
@@ -366,6 +366,22 @@ class AhAOp:
         nbytes = out.numel() * 8
         _memcpy_d2d(out.data_ptr(), ptr.value, nbytes)
         return out
+
+    def set_kernel_format(self, fmt: str) -> None:
+        """Kernel storage read by the applies: "fp32", "bf16" or "fp16"
+        (library entry mr_op_set_kernel_format; 16-bit entries with one FP32
+        scale per 4x4 block, for the shapes with a compiled fused x pass).
+        No counterpart in the reference."""
+        self._check_open()
+        code = {"fp32": KERNEL_FP32, "bf16": KERNEL_BF16, "fp16": KERNEL_FP16}.get(fmt)
+        if code is None:
+            raise ValueError(f"kernel format {fmt!r}: expected 'fp32', 'bf16' or 'fp16'")
+        check(lib.mr_op_set_kernel_format(self._handle, code), "mr_op_set_kernel_format")
+        self._kfmt = fmt
+
+    @property
+    def kernel_format(self) -> str:
+        return getattr(self, "_kfmt", "fp32")
 
     def cg(self, b, x=None, *, lam: float = 0.0, iters: int = 40, refresh: int = 10,
            op_refresh: Optional["AhAOp"] = None, graph: bool = False):

@@ -614,6 +614,9 @@ struct mr_aha_plan_s {
     size_t xbd_lds;     // its dynamic LDS bytes
     mr_fdiv nxdiv;
     mr_xbd_ct_fn xbd_ct;  // kernel specialised for (Mx, L), or null: mr_k_xbd
+    int kf;               // kernel format used by execute (MR_KF_*)
+    const unsigned *kq;   // 16-bit entries and block scales (kf != FP32)
+    const float *ksc;
 };
 
 static int mr_prof_mark(mr_aha_plan p, int stage)
@@ -739,10 +742,12 @@ static int mr_xbd_setup(mr_aha_plan p)
     int ct_gmax = 0;
     mr_xbd_ct_fn ct = nullptr;
     if (mr_env_ll("MR_AHA_HIP_XBD_CT", 1) != 0)
-        ct = mr_xbd_ct_pick(Mx, L, cfg.is_masked != 0, &ct_gmax);
+        ct = mr_xbd_ct_pick(Mx, L, cfg.is_masked != 0, p->kf, &ct_gmax);
     if (ct) {
         fn = (const void *)ct;
         if (gmax > ct_gmax) gmax = ct_gmax;
+    } else if (p->kf != MR_KF_FP32) {
+        return MR_SUCCESS;
     } else {
         mr_xbd_fn gf = cfg.is_masked ? mr_xbd_pick<true>(L) : mr_xbd_pick<false>(L);
         if (!gf) return MR_SUCCESS;
@@ -828,6 +833,30 @@ static int mr_plan_create_impl(const mr_config *cfg, mr_aha_plan *out)
 }
 
 int mr_aha_plan_group(mr_aha_plan p) { return p->group; }
+
+// Switch the kernel format the plan's executions read: MR_KF_FP32 (the
+// d_phi2_toep argument) or a 16-bit format held at q, sc (mr_kf_quantize).
+// A 16-bit format needs the compiled fused x pass for this shape.
+int mr_aha_plan_set_kernel_format(mr_aha_plan p, int kf, const unsigned *q, const float *sc)
+{
+    const int old = p->kf;
+    p->kf = kf;
+    p->kq = q;
+    p->ksc = sc;
+    const int rc = mr_xbd_setup(p);
+    if (rc != MR_SUCCESS || (kf != MR_KF_FP32 && !p->xbd_ct)) {
+        p->kf = old;
+        p->kq = nullptr;
+        p->ksc = nullptr;
+        (void)mr_xbd_setup(p);
+        if (rc != MR_SUCCESS) return rc;
+        MR_FAIL(MR_ERR_UNSUPPORTED,
+                "16-bit kernel: no compiled fused x pass for x grid %d, L = %d%s (available: "
+                "256/L4, 320/L8, 640/L16, unmasked)",
+                p->dims.m[0], p->cfg.L, p->cfg.is_masked ? ", masked" : "");
+    }
+    return MR_SUCCESS;
+}
 extern "C" MR_EXPORT int mr_aha_plan_xbd_group(mr_aha_plan p)
 {
     mr_clear_error();
@@ -950,7 +979,8 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
                     blocks);
         if (p->xbd_ct) {
             hipLaunchKernelGGL(p->xbd_ct, dim3((unsigned)blocks), dim3(MR_XBD_THREADS), p->xbd_lds,
-                               s, B, ker, p->d_inv, kstride, d, p->fx.tws, R, G, nc, ncg);
+                               s, B, ker, p->d_inv, kstride, d, p->fx.tws, R, G, nc, ncg, p->kq,
+                               p->ksc);
             MR_LAUNCH_CHECK("mr_k_xbd_ct");
         } else {
             mr_xbd_fn fn = masked ? mr_xbd_pick<true>(L) : mr_xbd_pick<false>(L);

@@ -22,6 +22,9 @@ int mr_aha_plan_execute_raw(mr_aha_plan plan, const mr_complex *d_x, const mr_co
                             const mr_complex *d_phi2_toep, const int64_t *d_mask,
                             mr_complex *d_AhAx);
 int mr_aha_plan_group(mr_aha_plan p);
+int mr_aha_plan_set_kernel_format(mr_aha_plan p, int kf, const unsigned *q, const float *sc);
+int mr_kf_quantize(int kf, const float2 *ker, long long kcol, int L, int upper, unsigned *q,
+                   float *sc, hipStream_t stream);
 
 struct mr_nufft_eng;
 int mr_nufft_eng_create(const mr_config *cfg, const mr_complex *d_phi2, const float *d_traj,
@@ -43,6 +46,9 @@ struct mr_op_s {
     mr_complex *own_SEs;
     int64_t *own_mask;
     mr_complex *own_toep;
+    unsigned *own_kq;   // 16-bit kernel entries (mr_op_set_kernel_format)
+    float *own_ksc;     // and their block scales
+    int kf;
     mr_op_info_t info;
 };
 
@@ -53,6 +59,8 @@ static void mr_op_release(mr_op p)
     if (p->own_SEs) (void)hipFree(p->own_SEs);
     if (p->own_mask) (void)hipFree(p->own_mask);
     if (p->own_toep) (void)hipFree(p->own_toep);
+    if (p->own_kq) (void)hipFree(p->own_kq);
+    if (p->own_ksc) (void)hipFree(p->own_ksc);
     (void)hipGetLastError();
     delete p;
 }
@@ -371,6 +379,61 @@ int mr_op_stream_len(mr_op op, hipStream_t *s, long long *n, int *device)
     *n = (long long)mr_x_size(&op->cfg);
     *device = op->primary;
     return MR_SUCCESS;
+}
+
+extern "C" MR_EXPORT int mr_op_set_kernel_format(mr_op op, int format)
+{
+    mr_clear_error();
+    MR_REQUIRE(op != nullptr, "mr_op_set_kernel_format: op is NULL");
+    MR_REQUIRE(format == MR_KERNEL_FP32 || format == MR_KERNEL_BF16 || format == MR_KERNEL_FP16,
+               "mr_op_set_kernel_format: format %d is not MR_KERNEL_FP32, _BF16 or _FP16", format);
+    if (!op->plan)
+        MR_FAIL(MR_ERR_UNSUPPORTED, "mr_op_set_kernel_format: a NUFFT operator has no kernel");
+    int prev = 0;
+    MR_HIP_CHECK(hipGetDevice(&prev));
+    if (prev != op->primary) MR_HIP_CHECK(hipSetDevice(op->primary));
+    int rc = MR_SUCCESS;
+    unsigned *q = nullptr;
+    float *sc = nullptr;
+    do {
+        if (format != MR_KERNEL_FP32) {
+            const long long kcol = op->cfg.is_masked ? (long long)op->cfg.num_masked
+                                                     : (long long)mr_pad_size(&op->cfg);
+            const int L = op->cfg.L;
+            const long long T = (long long)L * (L + 1) / 2;
+            const long long NB = (long long)((L + 3) / 4) * ((L + 3) / 4 + 1) / 2;
+            if (hipMalloc((void **)&q, (size_t)(T * kcol) * 4) != hipSuccess ||
+                hipMalloc((void **)&sc, (size_t)(NB * kcol) * 4) != hipSuccess) {
+                (void)hipGetLastError();
+                mr_set_error("mr_op_set_kernel_format: failed to allocate the 16-bit kernel "
+                             "(%lld bytes)", (T + NB) * kcol * 4);
+                rc = MR_ERR_ALLOC;
+                break;
+            }
+            rc = mr_kf_quantize(format, reinterpret_cast<const float2 *>(op->d_toep), kcol, L,
+                                op->cfg.triangulation == MR_TRI_UPPER ? 1 : 0, q, sc, op->stream);
+            if (rc != MR_SUCCESS) break;
+        }
+        rc = mr_aha_plan_set_kernel_format(op->plan, format, q, sc);
+        if (rc != MR_SUCCESS) break;
+        // earlier applies may still read the previous buffers
+        if (hipStreamSynchronize(op->stream) != hipSuccess) {
+            mr_set_error("mr_op_set_kernel_format: stream synchronisation failed");
+            rc = MR_ERR_CUDA;
+            break;
+        }
+        if (op->own_kq) (void)hipFree(op->own_kq);
+        if (op->own_ksc) (void)hipFree(op->own_ksc);
+        op->own_kq = q;
+        op->own_ksc = sc;
+        q = nullptr;
+        sc = nullptr;
+        op->kf = format;
+    } while (false);
+    if (q) (void)hipFree(q);
+    if (sc) (void)hipFree(sc);
+    if (prev != op->primary) (void)hipSetDevice(prev);
+    return rc;
 }
 
 extern "C" MR_EXPORT int mr_op_xbd_group(mr_op op)

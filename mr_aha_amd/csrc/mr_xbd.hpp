@@ -102,9 +102,134 @@ __device__ __forceinline__ void mr_bd_walk_g(float2 *a, int ms, const float2 *ke
 }
 
 
-// Host side of the specialised kernels: the kernel for x length n, rank L
-// and mask mode, or nullptr when n is not one of the specialised lengths.
+// ---------------------------------------------------------------------------
+// 16-bit kernel format (BF16 or FP16 entries with FP32 scales per 4x4 block)
+// ---------------------------------------------------------------------------
+// Entry t of location k is stored as one 32-bit word q[t * kcol + k] (real
+// part in the low half, imaginary part in the high half), both divided by the
+// scale of its 4x4 block of the L x L matrix: sc[bi * kcol + k] = the largest
+// |real| or |imaginary| part of the block's stored entries (1 for an all-zero
+// block), bi = h (h + 1) / 2 + l for block row/column indices h >= l.
+#define MR_KF_FP32 0
+#define MR_KF_BF16 1
+#define MR_KF_FP16 2
+
+__host__ __device__ constexpr int mr_kf_nblk(int L)
+{
+    return ((L + 3) / 4) * ((L + 3) / 4 + 1) / 2;
+}
+
+__device__ __forceinline__ int mr_kf_blk(int r, int c)
+{
+    const int a = r >> 2, b = c >> 2;
+    const int h = a > b ? a : b, l = a > b ? b : a;
+    return h * (h + 1) / 2 + l;
+}
+
+template <int KF> __device__ __forceinline__ float2 mr_kf_decode(unsigned u)
+{
+    if constexpr (KF == MR_KF_BF16) {
+        return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xFFFF0000u));
+    } else {
+        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+        const h2 h = __builtin_bit_cast(h2, u);
+        return make_float2((float)h.x, (float)h.y);
+    }
+}
+
+__device__ __forceinline__ unsigned mr_bf16_rne(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+
+// Block multiply of one location from the 16-bit kernel: as mr_bd_walk_g,
+// with the entries of each 4-row run of a column (one block) accumulated
+// unscaled and the block's scale applied once: y_c += s * sum_r q_rc v_r,
+// y_r += conj(q_rc) (s v_c).
+template <int L, bool UPPER, int KF>
+__device__ __forceinline__ void mr_bd_walk_q(float2 *a, int ms, const unsigned *q, const float *sc,
+                                             long long ks, unsigned bq, unsigned bs)
+{
+    float2 y[L];
+    unsigned kk[2][L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) y[l] = make_float2(0.f, 0.f);
+    auto col = [](int i, int &c, int &t0, int &len) {
+        c = UPPER ? L - 1 - i : i;
+        t0 = UPPER ? c * (c + 1) / 2 : c * L - c * (c - 1) / 2;
+        len = UPPER ? c + 1 : L - c;
+    };
+    auto ldw = [&](const unsigned *base, unsigned off, unsigned soff) {
+        const __amdgpu_buffer_rsrc_t r = mr_rsrc(base, 0xFFFFFFFFu);
+        return (unsigned)__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, (int)soff, 0);
+    };
+    {
+        int c, t0, len;
+        col(0, c, t0, len);
+#pragma unroll
+        for (int j = 0; j < L; ++j)
+            if (j < len) kk[0][j] = ldw(q + ks * t0, bq, (unsigned)(ks * 4 * j));
+    }
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        int c, t0, len;
+        col(i, c, t0, len);
+        if (i + 1 < L) {
+            int c1, t1, len1;
+            col(i + 1, c1, t1, len1);
+#pragma unroll
+            for (int j = 0; j < L; ++j)
+                if (j < len1) kk[(i + 1) & 1][j] = ldw(q + ks * t1, bq, (unsigned)(ks * 4 * j));
+        }
+        // scales of the blocks this column crosses (rows r0 .. r0 + len - 1)
+        const int r0 = UPPER ? 0 : c;
+        float s[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int rb = (r0 >> 2) + b;
+            if (rb * 4 < r0 + len) {
+                const int bi = (rb > (c >> 2)) ? rb * (rb + 1) / 2 + (c >> 2)
+                                               : (c >> 2) * ((c >> 2) + 1) / 2 + rb;
+                s[b] = __uint_as_float(ldw((const unsigned *)sc, bs, (unsigned)(ks * 4 * bi)));
+            }
+        }
+        asm volatile("" ::: "memory");
+        const float2 vc = a[c * ms];
+        float2 part = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j = 0; j < L; ++j) {
+            if (j < len) {
+                const int r = r0 + j;
+                const int b = (r >> 2) - (r0 >> 2);
+                const float2 e = mr_kf_decode<KF>(kk[i & 1][j]);
+                const float2 vr = (r == c) ? vc : a[r * ms];
+                part = cmac(part, e, vr);
+                if (r != c) {
+                    const float2 vcs = make_float2(s[b] * vc.x, s[b] * vc.y);
+                    y[r] = cmacc(y[r], e, vcs);
+                }
+                // end of this block's run of rows: apply its scale to y_c
+                if (j == len - 1 || ((r + 1) & 3) == 0) {
+                    y[c].x = __builtin_fmaf(s[b], part.x, y[c].x);
+                    y[c].y = __builtin_fmaf(s[b], part.y, y[c].y);
+                    part = make_float2(0.f, 0.f);
+                }
+            }
+        }
+        a[c * ms] = y[c];
+    }
+}
+
+// Host side of the specialised kernels: the kernel for x length n, rank L,
+// mask mode and kernel format kf (MR_KF_*), or nullptr when there is none.
 // *gmax receives the most coils one workgroup may take.
 typedef void (*mr_xbd_ct_fn)(float2 *, const float2 *, const int *, long long, mr_dims,
-                             const float2 *, long long, int, int, int);
-mr_xbd_ct_fn mr_xbd_ct_pick(int n, int L, bool masked, int *gmax);
+                             const float2 *, long long, int, int, int, const unsigned *,
+                             const float *);
+mr_xbd_ct_fn mr_xbd_ct_pick(int n, int L, bool masked, int kf, int *gmax);
+
+// Fill the 16-bit format q [T, kcol], sc [NB, kcol] from an FP32 kernel.
+int mr_kf_quantize(int kf, const float2 *ker, long long kcol, int L, int upper, unsigned *q,
+                   float *sc, hipStream_t stream);

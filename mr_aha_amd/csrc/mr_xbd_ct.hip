@@ -28,10 +28,11 @@ constexpr int mr_xbd_ct_gmax(int N, int L)
     return g < 1 ? 1 : g;
 }
 
-template <int N, int L, bool MASKED>
+template <int N, int L, bool MASKED, int KF>
 __global__ void __launch_bounds__(MR_XBD_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
 mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__restrict__ inv,
-            long long kstride, mr_dims d, const float2 *tws, long long R, int G, int ncoil, int ncg)
+            long long kstride, mr_dims d, const float2 *tws, long long R, int G, int ncoil, int ncg,
+            const unsigned *__restrict__ kq, const float *__restrict__ ksc)
 {
     constexpr int NW = MR_XBD_THREADS / 64;
     constexpr int TPMAX = mr_xbd_ct_gmax(N, L) * L;
@@ -87,6 +88,11 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
         if (b < 0) {
 #pragma unroll
             for (int l = 0; l < L; ++l) a[l * N] = make_float2(0.f, 0.f);
+        } else if constexpr (KF != MR_KF_FP32) {
+            if (d.tri_upper)
+                mr_bd_walk_q<L, true, KF>(a, N, kq, ksc, kstride, (unsigned)(b * 4), (unsigned)(b * 4));
+            else
+                mr_bd_walk_q<L, false, KF>(a, N, kq, ksc, kstride, (unsigned)(b * 4), (unsigned)(b * 4));
         } else if (d.tri_upper) {
             mr_bd_walk_g<L, true>(a, N, ker, kstride, (unsigned)(b * 8));
         } else {
@@ -113,18 +119,79 @@ template <int N> mr_xbd_ct_fn mr_xbd_ct_pick_n(int L, bool masked, int *gmax)
 #define MR_CT_CASE(l)                                                          \
     case l:                                                                    \
         *gmax = mr_xbd_ct_gmax(N, l);                                          \
-        return masked ? mr_k_xbd_ct<N, l, true> : mr_k_xbd_ct<N, l, false>;
+        return masked ? mr_k_xbd_ct<N, l, true, MR_KF_FP32>                    \
+                      : mr_k_xbd_ct<N, l, false, MR_KF_FP32>;
         MR_CT_CASE(4) MR_CT_CASE(8) MR_CT_CASE(16)
 #undef MR_CT_CASE
     default: return nullptr;
     }
 }
 
+// 16-bit kernels: unmasked, (N, L) of the benchmark shapes.
+template <int KF> mr_xbd_ct_fn mr_xbd_ct_pick_q(int n, int L, int *gmax)
+{
+#define MR_CTQ(nn, l)                                                          \
+    if (n == nn && L == l) {                                                   \
+        *gmax = mr_xbd_ct_gmax(nn, l);                                         \
+        return mr_k_xbd_ct<nn, l, false, KF>;                                  \
+    }
+    MR_CTQ(256, 4) MR_CTQ(320, 8) MR_CTQ(640, 16)
+#undef MR_CTQ
+    return nullptr;
+}
+
+// Quantise an FP32 kernel [T, kcol] into the 16-bit format.
+template <int KF>
+__global__ void mr_k_quantize(const float2 *__restrict__ ker, long long kcol, int L, int upper,
+                              unsigned *__restrict__ q, float *__restrict__ sc)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= kcol) return;
+    float mx[10];
+    const int nb = mr_kf_nblk(L);
+    for (int b = 0; b < nb; ++b) mx[b] = 0.f;
+    int t = 0;
+    for (int c = 0; c < L; ++c) {
+        const int r0 = upper ? 0 : c, r1 = upper ? c : L - 1;
+        for (int r = r0; r <= r1; ++r, ++t) {
+            const float2 e = ker[k + kcol * t];
+            const int b = mr_kf_blk(r, c);
+            mx[b] = fmaxf(mx[b], fmaxf(fabsf(e.x), fabsf(e.y)));
+        }
+    }
+    for (int b = 0; b < nb; ++b) {
+        if (!(mx[b] > 0.f)) mx[b] = 1.f;
+        sc[k + kcol * b] = mx[b];
+    }
+    t = 0;
+    for (int c = 0; c < L; ++c) {
+        const int r0 = upper ? 0 : c, r1 = upper ? c : L - 1;
+        for (int r = r0; r <= r1; ++r, ++t) {
+            const float2 e = ker[k + kcol * t];
+            const float inv = 1.f / mx[mr_kf_blk(r, c)];
+            const float re = e.x * inv, im = e.y * inv;
+            unsigned w;
+            if (KF == MR_KF_BF16) {
+                w = mr_bf16_rne(re) | (mr_bf16_rne(im) << 16);
+            } else {
+                typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+                h2 h;
+                h.x = (_Float16)re;
+                h.y = (_Float16)im;
+                w = __builtin_bit_cast(unsigned, h);
+            }
+            q[k + kcol * t] = w;
+        }
+    }
+}
+
 }  // namespace
 
-mr_xbd_ct_fn mr_xbd_ct_pick(int n, int L, bool masked, int *gmax)
+mr_xbd_ct_fn mr_xbd_ct_pick(int n, int L, bool masked, int kf, int *gmax)
 {
     *gmax = 0;
+    if (kf == MR_KF_BF16) return masked ? nullptr : mr_xbd_ct_pick_q<MR_KF_BF16>(n, L, gmax);
+    if (kf == MR_KF_FP16) return masked ? nullptr : mr_xbd_ct_pick_q<MR_KF_FP16>(n, L, gmax);
     switch (n) {
     case 256: return mr_xbd_ct_pick_n<256>(L, masked, gmax);
     case 320: return mr_xbd_ct_pick_n<320>(L, masked, gmax);
@@ -133,4 +200,19 @@ mr_xbd_ct_fn mr_xbd_ct_pick(int n, int L, bool masked, int *gmax)
     case 640: return mr_xbd_ct_pick_n<640>(L, masked, gmax);
     default: return nullptr;
     }
+}
+
+int mr_kf_quantize(int kf, const float2 *ker, long long kcol, int L, int upper, unsigned *q,
+                   float *sc, hipStream_t stream)
+{
+    MR_REQUIRE(L >= 1 && L <= 16, "16-bit kernel format: L = %d outside [1, 16]", L);
+    const unsigned blocks = (unsigned)((kcol + 255) / 256);
+    if (kf == MR_KF_BF16)
+        hipLaunchKernelGGL(mr_k_quantize<MR_KF_BF16>, dim3(blocks), dim3(256), 0, stream, ker, kcol,
+                           L, upper, q, sc);
+    else
+        hipLaunchKernelGGL(mr_k_quantize<MR_KF_FP16>, dim3(blocks), dim3(256), 0, stream, ker, kcol,
+                           L, upper, q, sc);
+    MR_LAUNCH_CHECK("mr_k_quantize");
+    return MR_SUCCESS;
 }

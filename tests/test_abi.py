@@ -26,8 +26,9 @@ def declared_symbols():
 
 def test_exports_every_declared_symbol(L):
     syms = declared_symbols()
-    # 25 reference entry points + mr_version + 5 extensions (profile, pipeline, CG)
-    assert len(syms) == 31
+    # 25 reference entry points + mr_version + 6 extensions (profile, pipeline, CG,
+    # kernel format)
+    assert len(syms) == 32
     for s in syms:
         assert hasattr(L.lib, s), s
     out = subprocess.run(["nm", "-D", "--defined-only", str(L.LIB_PATH)], capture_output=True,
