@@ -40,7 +40,10 @@ MR_EXPORT int mr_op_xbd_group(mr_op op);
  * Every `refresh` iterations (0 = never) the recursive residual is replaced
  * by the true residual b - (A^H A + lambda I) x, computed with `op_refresh`
  * (same shape, device and stream; e.g. the FP32 operator when `op` holds a
- * 16-bit kernel) or with `op` when op_refresh is NULL. d_x holds the initial
+ * 16-bit kernel) or with `op` when op_refresh is NULL. A refresh through a
+ * different operator also restarts the search directions, so each cycle of
+ * `refresh` iterations solves with `op` for the correction of the residual
+ * of `op_refresh` (iterative refinement). d_x holds the initial
  * guess (or is zeroed with MR_CG_ZERO_INIT) and receives the solution.
  * res_hist (host, optional) receives ||r_k|| / ||b|| for k = 0..n_iter, r_k
  * being the residual the iteration carries. MR_CG_GRAPH captures one

@@ -165,6 +165,7 @@ __global__ void __launch_bounds__(CG_THREADS) k_zero(float2 *__restrict__ x, lon
 //   1: rr  = sum, hist[k] = rr              (initial residual)
 //   2: pAp = Re sum, alpha = rr / pAp
 //   3: beta = sum / rr, rr = sum, hist[k] = rr
+//   4: beta = 0 (restart), rr = sum, hist[k] = rr
 __global__ void __launch_bounds__(CG_THREADS)
 k_finish(const double2 *__restrict__ part, int nparts, double *scal, double *hist, int k, int mode)
 {
@@ -188,7 +189,7 @@ k_finish(const double2 *__restrict__ part, int nparts, double *scal, double *his
         scal[S_ALPHA] = (s > 0.0) ? scal[S_RR] / s : 0.0;
     } else {
         const double rr = scal[S_RR];
-        scal[S_BETA] = (rr > 0.0) ? s / rr : 0.0;
+        scal[S_BETA] = (mode == 3 && rr > 0.0) ? s / rr : 0.0;
         scal[S_RR] = s;
         hist[k] = s;
     }
@@ -217,6 +218,9 @@ unsigned grid_for(long long n)
 
 // One iteration's work after the apply of p (Ap holds A^H A p):
 // refresh = false: recursive residual; true: x update, then the true residual.
+// A refresh through a different operator restarts the directions (beta = 0):
+// each cycle then solves for the correction of the refreshed residual with
+// `op`, i.e. iterative refinement of op's precision against opr's.
 int cg_iter(mr_op op, mr_op opr, hipStream_t s, const cg_ws &w, const float2 *b, float2 *x,
             float lam, long long n, int k, bool refresh)
 {
@@ -238,7 +242,7 @@ int cg_iter(mr_op op, mr_op opr, hipStream_t s, const cg_ws &w, const float2 *b,
                            w.part);
     }
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(CG_THREADS), 0, s, w.part, (int)G, w.scal, w.hist,
-                       k + 1, 3);
+                       k + 1, (refresh && opr != op) ? 4 : 3);
     hipLaunchKernelGGL(k_pupd, dim3(G), dim3(CG_THREADS), 0, s, w.r, w.p, w.scal, n);
     MR_LAUNCH_CHECK("cg iteration");
     return MR_SUCCESS;
