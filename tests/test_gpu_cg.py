@@ -121,15 +121,21 @@ def test_cg_graph_matches_eager(mr):
 
 
 def test_cg_refresh_with_second_operator(mr):
-    """op_refresh: the residual replacement goes through the second operator;
-    with identical operators the result equals refresh through the first."""
+    """op_refresh = op itself is the plain residual replacement (bit-identical
+    to no op_refresh); a separate but identical operator restarts each cycle
+    from its true residual and converges to the same solution."""
     g, S, op, xt = _problem(mr)
     _, _, op2, _ = _problem(mr)
     with op, op2:
         b = op(dev(xt))
-        x1, h1 = op.cg(b, lam=1e-2, iters=10, refresh=5)
-        x2, h2 = op.cg(b, lam=1e-2, iters=10, refresh=5, op_refresh=op2)
-        assert torch.equal(x1, x2) and h1 == h2
+        lam = 0.5 * float(torch.linalg.vector_norm(b) / torch.linalg.vector_norm(dev(xt)))
+        b = (b + lam * dev(xt)).contiguous()
+        x1, h1 = op.cg(b, lam=lam, iters=120, refresh=10)
+        x0, h0 = op.cg(b, lam=lam, iters=120, refresh=10, op_refresh=op)
+        assert torch.equal(x0, x1) and h0 == h1
+        x2, h2 = op.cg(b, lam=lam, iters=120, refresh=10, op_refresh=op2)
+        assert h2[-1] < 1e-4 * h2[0]
+        assert float(torch.linalg.vector_norm(x2 - x1) / torch.linalg.vector_norm(x1)) < 1e-4
 
 
 def test_cg_argument_errors(mr):
