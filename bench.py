@@ -30,7 +30,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 
-def stage_bytes(cfg, nc, xbd):
+def stage_bytes(cfg, nc, xbd, precision="fp32"):
     """Algorithmic HBM bytes of one launch of each profiled stage for a group
     of nc coils: every array a stage must read or write, counted once (the
     kernel once per launch, however many coil groups of a row share it).
@@ -50,7 +50,9 @@ def stage_bytes(cfg, nc, xbd):
     N = Nx * Ny * Nz
     P = Mx * My * Mz
     c8 = 8
-    kern = T * (cfg.num_masked if cfg.is_masked else P) * c8
+    kcol = cfg.num_masked if cfg.is_masked else P
+    nblk = ((L + 3) // 4) * ((L + 3) // 4 + 1) // 2
+    kern = T * kcol * c8 if precision == "fp32" else (T + nblk) * kcol * 4
     xs = N * L * c8 + N * nc * c8       # x and the maps, once
     outs = N * nc * c8 + N * L * c8     # maps and AhAx
     three = cfg.n_spatial == 3
@@ -105,14 +107,16 @@ def calibrate_copy(nbytes=1 << 30, reps=4):
           flush=True)
 
 
-def measured_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` on `workload` from the committed
-    rocprofv3 record (profiles/hbm_traffic.json), or None."""
+def measured_traffic(kernel, workload, precision="fp32"):
+    """HBM bytes per launch of `kernel` on `workload` with the kernel stored
+    in `precision`, from the committed rocprofv3 record
+    (profiles/hbm_traffic.json), or None."""
     path = ROOT / "profiles" / "hbm_traffic.json"
     if not path.exists():
         return None
     for rec in json.loads(path.read_text()).get("records", []):
-        if rec.get("kernel") == kernel and rec.get("workload") == workload:
+        if (rec.get("kernel") == kernel and rec.get("workload") == workload
+                and rec.get("precision", "fp32") == precision):
             return int(rec["hbm_bytes_per_launch"])
     return None
 
@@ -145,6 +149,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="stack", choices=["stack", "radial", "koosh"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16"],
+                    help="kernel storage read by the applies (16-bit: per-4x4-block scales)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -187,6 +193,8 @@ def main():
     print(f"[bench] operator created in {t_create:.1f} s (kernel {info['toep_ms'] / 1e3:.1f} s), "
           f"group {info['group'][0]}", file=sys.stderr, flush=True)
 
+    if args.precision != "fp32":
+        op.set_kernel_format(args.precision)
     x = w["x"]
     out = torch.empty_like(x)
     for _ in range(args.warmup):
@@ -215,7 +223,7 @@ def main():
 
     group = info["group"][0]
     xbd = int(_lib.lib.mr_op_xbd_group(op._handle))
-    sb = stage_bytes(cfg, group, xbd)
+    sb = stage_bytes(cfg, group, xbd, args.precision)
     stages = {}
     for i, name in enumerate(_lib.PROFILE_STAGES):
         if calls[i]:
@@ -241,15 +249,18 @@ def main():
         "value": round(value, 3), "unit": "A^H A applies/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c64 (fp32 complex)", "data": "synthetic",
-        "config": {"workload": workload, "precision": "fp32 kernel", "coil_group": group,
+        "dtype": "c64 (fp32 complex)" if args.precision == "fp32"
+                 else f"c64 vectors, {args.precision} kernel",
+        "data": "synthetic",
+        "config": {"workload": workload, "precision": f"{args.precision} kernel",
+                   "coil_group": group,
                    "pipeline": (f"fused x pass, {xbd} coil(s) per workgroup" if xbd
                                 else "separate FFT and block passes"),
                    "kernel_build_s": round(info["toep_ms"] / 1e3, 2)},
         "roofline": {"bound": "hbm", "kernel": dom_kernel, "stage": dom,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(dom_kernel, workload),
+                     "traffic": measured_traffic(dom_kernel, workload, args.precision),
                      "algorithmic_bytes_per_launch": sb[dom]},
         "apply_model": {"bytes": B, "gbs": round(B / (ms_per_step * 1e6), 1),
                         "frac": round(B / (ms_per_step * 1e6) / HBM_PEAK_GBS, 4)},
