@@ -58,8 +58,8 @@ MR_EXPORT int mr_op_xbd_group(mr_op op);
  * fused x pass reads that instead: half the kernel bytes per apply, at the
  * precision of the format (relative entry error 2^-9 BF16, 2^-11 FP16). The
  * FP32 kernel is kept: MR_KERNEL_FP32 switches back. Available where the
- * fused x pass is compiled for the shape (x grid 256 with L = 4, 320 with
- * L = 8, 640 with L = 16; unmasked); MR_ERR_UNSUPPORTED otherwise. Pair with
+ * fused x pass is compiled for the shape (x grid 256, 320, 384, 512 or 640
+ * with L = 4, 8 or 16; unmasked); MR_ERR_UNSUPPORTED otherwise. Pair with
  * mr_op_cg's op_refresh = an FP32 operator for iterative refinement. No
  * counterpart in the reference. */
 #define MR_KERNEL_FP32 0

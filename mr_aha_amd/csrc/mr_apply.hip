@@ -852,7 +852,7 @@ int mr_aha_plan_set_kernel_format(mr_aha_plan p, int kf, const unsigned *q, cons
         if (rc != MR_SUCCESS) return rc;
         MR_FAIL(MR_ERR_UNSUPPORTED,
                 "16-bit kernel: no compiled fused x pass for x grid %d, L = %d%s (available: "
-                "256/L4, 320/L8, 640/L16, unmasked)",
+                "x grid 256, 320, 384, 512 or 640 with L 4, 8 or 16, unmasked)",
                 p->dims.m[0], p->cfg.L, p->cfg.is_masked ? ", masked" : "");
     }
     return MR_SUCCESS;

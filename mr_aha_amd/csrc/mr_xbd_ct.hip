@@ -127,7 +127,7 @@ template <int N> mr_xbd_ct_fn mr_xbd_ct_pick_n(int L, bool masked, int *gmax)
     }
 }
 
-// 16-bit kernels: unmasked, (N, L) of the benchmark shapes.
+// 16-bit kernels: unmasked, every compiled (N, L).
 template <int KF> mr_xbd_ct_fn mr_xbd_ct_pick_q(int n, int L, int *gmax)
 {
 #define MR_CTQ(nn, l)                                                          \
@@ -135,7 +135,9 @@ template <int KF> mr_xbd_ct_fn mr_xbd_ct_pick_q(int n, int L, int *gmax)
         *gmax = mr_xbd_ct_gmax(nn, l);                                         \
         return mr_k_xbd_ct<nn, l, false, KF>;                                  \
     }
-    MR_CTQ(256, 4) MR_CTQ(320, 8) MR_CTQ(640, 16)
+    MR_CTQ(256, 4) MR_CTQ(256, 8) MR_CTQ(256, 16) MR_CTQ(320, 4) MR_CTQ(320, 8)
+    MR_CTQ(320, 16) MR_CTQ(384, 4) MR_CTQ(384, 8) MR_CTQ(384, 16) MR_CTQ(512, 4)
+    MR_CTQ(512, 8) MR_CTQ(512, 16) MR_CTQ(640, 4) MR_CTQ(640, 8) MR_CTQ(640, 16)
 #undef MR_CTQ
     return nullptr;
 }

@@ -1,0 +1,73 @@
+"""GPU test of the continuous-batching server (mr_aha_amd.serve): every
+request's solution is the same when served alone and when served among
+overlapping requests of other shapes (tolerance 1e-6 relative: each request
+runs the same kernels in the same order on its own stream, so equality is
+expected to be exact), and admission respects the slot and memory limits."""
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def serve():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from mr_aha_amd import serve
+    return serve
+
+
+SHAPES = [("2D", 64, 4, 3, "fp32"), ("2D", 96, 8, 4, "fp32"), ("stack", 48, 4, 2, "fp32"),
+          ("2D", 128, 4, 4, "bf16"), ("2D", 80, 16, 2, "fp32"), ("stack", 40, 8, 3, "fp32")]
+
+
+def _requests(serve):
+    from mr_aha_amd import workloads as wl
+    reqs = []
+    for i, (tt, n, L, C, prec) in enumerate(SHAPES):
+        if tt == "2D":
+            w = wl.radial_2d(n=n, L=L, C=C, n_spokes=2 * n, nkx=2 * n, n_frames=32, seed=100 + i)
+            nd = (n, n)
+        else:
+            w = wl.stack_of_stars(n=n, nz=8, L=L, C=C, n_spokes=n, nkx=2 * n, n_frames=32,
+                                  seed=100 + i)
+            nd = (n, n, 8)
+        b = wl.complex_gaussian(tuple(w["x"].shape), 500 + i, "cuda")
+        reqs.append(serve.Request(rid=i, traj_type=tt, n_dim=nd, L=L, n_coils=C, SEs=w["SEs"],
+                                  phi2=w["phi2"], traj=w["traj"], b=b, iters=12, lam=1e-2,
+                                  refresh=4, precision=prec, arrival=0.02 * i))
+    return reqs
+
+
+def test_same_alone_and_in_mix(serve):
+    alone = serve.Server(slots=1).run(_requests(serve), realtime=False)
+    mix = serve.Server(slots=3).run(_requests(serve), realtime=True)
+    for a, m in zip(alone, mix):
+        assert a.error is None and m.error is None, (a.error, m.error)
+        d = float(torch.linalg.vector_norm(a.x - m.x) / torch.linalg.vector_norm(a.x))
+        assert d < 1e-6, (a.rid, d)
+        assert a.hist[-1] < a.hist[0]
+        assert m.t_admit >= m.arrival - 1e-3 and m.t_done > m.t_admit
+    s = serve.summary(mix)
+    assert s["completed"] == len(SHAPES) and s["p99_s"] >= s["p50_s"] > 0
+
+
+def test_admission_limits(serve):
+    reqs = _requests(serve)
+    fp = [serve.footprint_bytes(r) for r in reqs]
+    # a budget that holds the largest request alone but not two of them
+    srv = serve.Server(budget_bytes=max(fp) + min(fp) // 2, slots=4)
+    out = srv.run(reqs, realtime=False)
+    assert all(r.error is None for r in out)
+    # no two requests whose footprints exceed the budget overlapped
+    for i, a in enumerate(out):
+        for b in out[i + 1:]:
+            overlap = a.t_admit < b.t_done and b.t_admit < a.t_done
+            if overlap:
+                assert serve.footprint_bytes(a) + serve.footprint_bytes(b) <= srv.budget
+    # a request larger than the budget is refused, the others still served
+    small = serve.Server(budget_bytes=min(fp), slots=2).run(_requests(serve), realtime=False)
+    assert any(r.error and "exceeds the budget" in r.error for r in small)
+    assert all(r.x is not None for r in small if r.error is None)
