@@ -142,12 +142,133 @@ def cpu_baseline(w, K, cfg, budget_s=30.0):
                       f"{dt:.1f} s, scaled x{C}"}
 
 
+def bench_cg(args):
+    """CG reconstructions of the 128x128 L=4 C=4 radial problem: 40 iterations,
+    true residual every 10, FP32 operator vs 16-bit kernel refined against it,
+    HIP-graph-replayed iterations. Error of each against the FP32 solution
+    converged to a 1e-6 relative residual (200 iterations)."""
+    import torch
+    import mr_aha_amd as mr
+    from mr_aha_amd import workloads
+    w = workloads.radial_2d(device="cuda")
+    s = torch.cuda.Stream()
+    cfg = mr.make_config(w["traj_type"], w["n_dim"], w["L"], n_coils=w["n_coils"], stream=s)
+    res = {}
+    with torch.cuda.stream(s):
+        K = mr.calc_toep(cfg, w["phi2"], w["traj"])
+        op32 = mr.create_op(cfg, w["SEs"], None, None, _phi2_toep=K)
+        op16 = mr.create_op(cfg, w["SEs"], None, None, _phi2_toep=K)
+        op16.set_kernel_format("bf16")
+        # smooth object: low-pass complex Gaussian coefficients
+        xt = workloads.complex_gaussian(tuple(w["x"].shape), 77, "cuda")
+        f = torch.fft.fftn(xt, dim=(-2, -1))
+        ky = torch.fft.fftfreq(xt.shape[-2], device="cuda").abs()[:, None] < 0.1
+        kx = torch.fft.fftfreq(xt.shape[-1], device="cuda").abs()[None, :] < 0.1
+        xt = torch.fft.ifftn(f * (ky & kx), dim=(-2, -1)).to(torch.complex64).contiguous()
+        Axt = op32(xt)
+        lam = 1e-3 * float(torch.linalg.vector_norm(Axt) / torch.linalg.vector_norm(xt))
+        b = (Axt + lam * xt).contiguous()
+        xref, href = op32.cg(b, lam=lam, iters=200, refresh=10, graph=True)
+        for name, op, ref in (("fp32", op32, None), ("bf16", op16, op32)):
+            for _ in range(args.warmup):
+                op.cg(b, lam=lam, iters=40, refresh=10, op_refresh=ref, graph=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                x, h = op.cg(b, lam=lam, iters=40, refresh=10, op_refresh=ref, graph=True)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / args.steps
+            err = float(torch.linalg.vector_norm(x - xref) / torch.linalg.vector_norm(xref))
+            res[name] = {"recon_per_s": round(1.0 / dt, 3), "ms_per_recon": round(dt * 1e3, 3),
+                         "final_residual": h[-1], "error_vs_converged_fp32": err}
+        op16.close()
+        op32.close()
+    line = {"metric": "CG reconstructions/s, 128x128 L4 C4 radial, 40 iterations (FP32 vs bf16 "
+                      "kernel refined against FP32)",
+            "value": res["bf16"]["recon_per_s"], "unit": "reconstructions/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+            "dtype": "c64 vectors, bf16 kernel, fp64 CG scalars", "data": "synthetic",
+            "config": {"workload": "2D radial 128x128, grid 256x256, L=4, C=4, 64x256 spokes; "
+                                   "CG 40 iterations, refresh 10, lambda 1e-3 |A|"},
+            "fp32": res["fp32"], "bf16": res["bf16"],
+            "speedup_mixed_vs_fp32": round(res["bf16"]["recon_per_s"] / res["fp32"]["recon_per_s"], 3),
+            "reference_residual": href[-1]}
+    print(json.dumps(line), flush=True)
+
+
+def bench_serve(args):
+    """Poisson mix of CG reconstruction requests (shapes of the serving
+    configuration) through mr_aha_amd.serve, FP32 and then 16-bit kernels."""
+    import random
+    import torch
+    from mr_aha_amd import serve, workloads
+    rnd = random.Random(2026)
+    specs = []
+    t = 0.0
+    for i in range(args.requests):
+        t += rnd.expovariate(4.0)               # 4 requests/s on average
+        if rnd.random() < 0.5:
+            n = rnd.choice([128, 192, 256, 320])
+            spec = ("2D", (n, n))
+        else:
+            n = rnd.choice([192, 256])
+            spec = ("stack", (n, n, rnd.choice([32, 48])))
+        L = rnd.choice([4, 8, 16])
+        C = rnd.choice([8, 12, 16])
+        ns = rnd.randint(256, 512)
+        specs.append((i, spec, L, C, ns, rnd.randint(20, 50), t))
+
+    def make(prec):
+        reqs = []
+        for i, (tt, nd), L, C, ns, it, arr in specs:
+            n = nd[0]
+            if tt == "2D":
+                w = workloads.radial_2d(n=n, L=L, C=C, n_spokes=ns, nkx=2 * n, n_frames=64,
+                                        seed=900 + i)
+            else:
+                w = workloads.stack_of_stars(n=n, nz=nd[2], L=L, C=C, n_spokes=ns, nkx=2 * n,
+                                             n_frames=64, seed=900 + i)
+            b = workloads.complex_gaussian(tuple(w["x"].shape), 1900 + i, "cuda")
+            reqs.append(serve.Request(rid=i, traj_type=tt, n_dim=nd, L=L, n_coils=C,
+                                      SEs=w["SEs"], phi2=w["phi2"], traj=w["traj"], b=b,
+                                      iters=it, lam=1e-2, refresh=10, precision=prec,
+                                      arrival=arr))
+        return reqs
+
+    out = {}
+    sols = {}
+    for prec in ("fp32", "bf16"):
+        reqs = make(prec)
+        torch.cuda.synchronize()
+        done = serve.Server(slots=4).run(reqs, realtime=True)
+        out[prec] = serve.summary(done)
+        out[prec]["errors"] = sorted({r.error for r in done if r.error})[:3]
+        sols[prec] = {r.rid: r.x for r in done if r.x is not None}
+        print(f"[serve] {prec}: {out[prec]}", file=sys.stderr, flush=True)
+        del reqs, done
+        torch.cuda.empty_cache()
+    diffs = [float(torch.linalg.vector_norm(sols["bf16"][k] - v) / torch.linalg.vector_norm(v))
+             for k, v in sols["fp32"].items() if k in sols["bf16"]]
+    line = {"metric": "serving: CG reconstruction requests/s and p50/p99 latency (Poisson mix)",
+            "value": round(out["fp32"]["requests_per_s"], 3), "unit": "requests/s", "n_gpus": 1,
+            "higher_is_better": True, "data": "synthetic",
+            "config": {"workload": f"{args.requests} Poisson requests at 4/s: 2D radial "
+                                   "{128,192,256,320}^2 or stack {192,256}^2 x {32,48}; L {4,8,16}; "
+                                   "C {8,12,16}; 256-512 spokes of 2N; 20-50 CG iterations; 4 slots"},
+            "fp32": out["fp32"], "bf16": out["bf16"],
+            "max_rel_diff_bf16_vs_fp32": max(diffs) if diffs else None}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="stack", choices=["stack", "radial", "koosh"])
+    ap.add_argument("--config", default="stack", choices=["stack", "radial", "koosh", "cg", "serve"],
+                    help="stack/radial/koosh: applies/s; cg: CG reconstructions/s, FP32 vs "
+                         "16-bit kernel; serve: request mix, p50/p99 latency")
+    ap.add_argument("--requests", type=int, default=24, help="serve: number of requests")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16"],
                     help="kernel storage read by the applies (16-bit: per-4x4-block scales)")
@@ -169,6 +290,10 @@ def main():
 
     torch.cuda.set_device(0)
     dev = "cuda:0"
+    if args.config == "cg":
+        return bench_cg(args)
+    if args.config == "serve":
+        return bench_serve(args)
     t_gen = time.perf_counter()
     if args.config == "stack":
         w = workloads.stack_of_stars(device=dev)
