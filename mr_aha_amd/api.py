@@ -435,7 +435,7 @@ class AhAOp:
 
 def _memcpy_d2d(dst: int, src: int, nbytes: int) -> None:
     from ._hiprt import memcpy_d2d
-    memcpy_d2d(dst, src, nbytes)
+    memcpy_d2d(dst, src, nbytes, int(torch.cuda.current_stream().cuda_stream))
 
 
 def create_op(cfg: MRConfig, SEs, phi2, traj, mask=None, **kwargs) -> AhAOp:

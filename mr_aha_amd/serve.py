@@ -8,9 +8,9 @@ normal equations and its memory until it finishes. The server:
   estimated HBM footprint (kernel, 16-bit copy, plan work buffers, CG
   vectors, maps) fits the remaining budget, so requests that arrive while
   others run start as soon as memory and a slot free up;
-* runs every admitted request on its own HIP stream from a worker thread (the
-  library calls release the GIL), so kernel builds, applies and CG
-  iterations of different requests overlap on the device;
+* runs every admitted request on its own non-blocking HIP stream from a
+  worker thread (the library calls release the GIL), so kernel builds,
+  applies and CG iterations of different requests overlap on the device;
 * solves with mr_op_cg: FP64 scalars on the device, HIP-graph-replayed
   iterations, and for a 16-bit kernel ("bf16"/"fp16") cycles restarted from
   the true residual of an FP32 operator sharing the same kernel memory;
@@ -31,7 +31,7 @@ from typing import List, Optional
 
 import torch
 
-from . import api
+from . import _hiprt, api
 
 
 @dataclass
@@ -100,7 +100,10 @@ class Server:
         self._running = 0
 
     def _solve(self, req: Request, t0: float) -> None:
-        stream = torch.cuda.Stream()
+        # a non-blocking stream: the graph capture of one request must not be
+        # joined by legacy-stream work that another request's thread issues
+        handle = _hiprt.stream_create_nonblocking()
+        stream = torch.cuda.ExternalStream(handle)
         try:
             with torch.cuda.stream(stream):
                 cfg = api.make_config(req.traj_type, req.n_dim, req.L, n_coils=req.n_coils,
@@ -124,11 +127,15 @@ class Server:
                     del K
         except Exception as e:  # reported per request
             req.error = f"{type(e).__name__}: {e}"
+        finally:
+            stream.synchronize()
+            _hiprt.stream_destroy(handle)
         req.t_done = time.perf_counter() - t0
 
     def run(self, requests: List[Request], realtime: bool = True) -> List[Request]:
         """Serve `requests` (arrival times in seconds; realtime=False admits
         them all at t = 0) and return them with their results and times."""
+        torch.cuda.synchronize()   # inputs written on other streams are complete
         order = sorted(requests, key=lambda r: r.arrival)
         for r in order:
             if footprint_bytes(r) > self.budget:
