@@ -83,8 +83,12 @@ int mr_nufft_eng_create(const mr_config *cfg, const mr_complex *d_phi2, const fl
     for (int a = 0; a < 3 && he == hipSuccess; ++a) {
         std::vector<double> dc = mr_grid_deconv(e->g, a);
         he = hipMalloc((void **)&e->dc[a], dc.size() * sizeof(double));
+        // stream-ordered (never the legacy stream, which could tie this call
+        // to another thread's graph capture); synchronised below
         if (he == hipSuccess)
-            he = hipMemcpy(e->dc[a], dc.data(), dc.size() * sizeof(double), hipMemcpyHostToDevice);
+            he = hipMemcpyAsync(e->dc[a], dc.data(), dc.size() * sizeof(double),
+                                hipMemcpyHostToDevice, e->s);
+        if (he == hipSuccess) he = hipStreamSynchronize(e->s);
     }
     if (he == hipSuccess)
         he = hipMemcpyAsync(e->phi2, d_phi2, phi2_bytes, hipMemcpyDeviceToDevice, e->s);

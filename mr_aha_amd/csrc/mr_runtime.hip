@@ -60,11 +60,24 @@ template <typename R> int mr_fft_get(int n, mr_fft_len<R> *out)
                 for (int k = 0; k < Ns; ++k) hs[r * Ns + k - 1] = h[(long long)k * r * stride];
             Ns *= rad;
         }
+        // Uploaded on a private non-blocking stream: a legacy-stream copy
+        // would be ordered after every blocking stream, including one that
+        // another thread is capturing into a graph.
         void *d = nullptr;
+        hipStream_t up = nullptr;
         MR_HIP_CHECK(hipMalloc(&d, 2 * n * sizeof(V)));
-        MR_HIP_CHECK(hipMemcpy(d, h.data(), n * sizeof(V), hipMemcpyHostToDevice));
-        MR_HIP_CHECK(hipMemcpy(reinterpret_cast<V *>(d) + n, hs.data(), n * sizeof(V),
-                               hipMemcpyHostToDevice));
+        MR_HIP_CHECK(hipStreamCreateWithFlags(&up, hipStreamNonBlocking));
+        hipError_t e = hipMemcpyAsync(d, h.data(), n * sizeof(V), hipMemcpyHostToDevice, up);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(reinterpret_cast<V *>(d) + n, hs.data(), n * sizeof(V),
+                               hipMemcpyHostToDevice, up);
+        if (e == hipSuccess) e = hipStreamSynchronize(up);
+        (void)hipStreamDestroy(up);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            MR_FAIL(MR_ERR_CUDA, "uploading the twiddles of length %d failed: %s", n,
+                    hipGetErrorString(e));
+        }
         it = g_tw.emplace(key, d).first;
     }
     out->tw = reinterpret_cast<const V *>(it->second);

@@ -228,7 +228,10 @@ int mr_calc_toep_impl(const mr_config *cfg, const mr_complex *d_phi2, const floa
     for (int a = 0; a < 3; ++a) {
         std::vector<double> dc = mr_grid_deconv(g, a);
         MR_HIP_CHECK(hipMalloc((void **)&b.dc[a], dc.size() * sizeof(double)));
-        MR_HIP_CHECK(hipMemcpy(b.dc[a], dc.data(), dc.size() * sizeof(double), hipMemcpyHostToDevice));
+        // stream-ordered, not on the legacy stream (see mr_fft_get)
+        MR_HIP_CHECK(hipMemcpyAsync(b.dc[a], dc.data(), dc.size() * sizeof(double),
+                                    hipMemcpyHostToDevice, s));
+        MR_HIP_CHECK(hipStreamSynchronize(s));
     }
     if (cfg->is_masked) MR_TRY(mr_build_mask_rows(cfg, d_mask, s, &b.inv));
 
