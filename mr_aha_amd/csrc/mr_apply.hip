@@ -485,7 +485,9 @@ __device__ __forceinline__ void mr_xbd_fft(float2 *A, const mr_fft_len<float> &f
         case 3: mr_stage_reg<3, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tws, f.n, f.n, tp, Ns, md, nd); break;
         case 4: mr_stage_reg<4, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tws, f.n, f.n, tp, Ns, md, nd); break;
         case 5: mr_stage_reg<5, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tws, f.n, f.n, tp, Ns, md, nd); break;
+        case 6: mr_stage_reg<6, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tws, f.n, f.n, tp, Ns, md, nd); break;
         case 7: mr_stage_reg<7, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tws, f.n, f.n, tp, Ns, md, nd); break;
+        case 10: mr_stage_reg<10, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tws, f.n, f.n, tp, Ns, md, nd); break;
         default: mr_stage_reg<8, INV, float2, MR_XBD_THREADS, MR_XBD_EMAX>(A, f.tws, f.n, f.n, tp, Ns, md, nd); break;
         }
         Ns *= RR;
@@ -724,7 +726,7 @@ static int mr_xbd_setup(mr_aha_plan p)
     const int L = cfg.L, Mx = p->dims.m[0];
     if (mr_env_ll("MR_AHA_HIP_XBD", 1) == 0 || L > 16) return MR_SUCCESS;
     for (int i = 0; i < p->fx.nst; ++i)
-        if (p->fx.radix[i] > 8 || p->fx.radix[i] == 6) return MR_SUCCESS;
+        if (!mr_fft_unrolled(p->fx.radix[i])) return MR_SUCCESS;
     const long long per_elems = (long long)L * Mx;
     const size_t per = (size_t)per_elems * sizeof(float2);
     if (per_elems > MR_XBD_EMAX) return MR_SUCCESS;

@@ -8,7 +8,9 @@
 //   B[(j div Ns) Ns R + k + q Ns] = V_q
 // (Govindaraju et al., "High performance discrete Fourier transforms on
 // graphics processors", SC 2008). Radices 2, 3, 4, 5, 7, 8 have unrolled
-// butterflies; any other prime factor uses a per-output dot product over
+// butterflies, and 6 and 10 prime-factor ones (no internal twiddles, so
+// e.g. 640 = 8 * 8 * 10 takes three stages); any other prime factor uses a
+// per-output dot product over
 // the n-th-root table, so every length works (prime grids of the test
 // fixtures) while smooth lengths take the fast butterflies.
 //
@@ -129,19 +131,22 @@ template <typename R> inline void mr_fft_fill_divs(mr_fft_len<R> *f)
     f->ndiv = mr_fdiv_make((unsigned)f->n);
 }
 
-// Factor n into the stage radices: 8s first, then 4, 2, 3, 5, 7, then the
-// remaining primes in ascending order.
+// Factor n into the stage radices: 8s first, then 10 and 6 (prime-factor
+// butterflies), 4, 2, 3, 5, 7, then the remaining primes in ascending order.
 inline int mr_fft_factor(int n, int *radix)
 {
     int ns = 0;
     int m = n;
-    const int small[] = {8, 4, 2, 3, 5, 7};
+    const int small[] = {8, 10, 6, 4, 2, 3, 5, 7};
     for (int r : small)
         while (m % r == 0 && ns < MR_FFT_MAXST) { radix[ns++] = r; m /= r; }
     for (int p = 11; m > 1 && ns < MR_FFT_MAXST; p += 2)
         while (m % p == 0 && ns < MR_FFT_MAXST) { radix[ns++] = p; m /= p; }
     return (m == 1) ? ns : -1;
 }
+
+// Radices with an unrolled butterfly.
+inline bool mr_fft_unrolled(int r) { return (r >= 2 && r <= 8) || r == 10; }
 
 // n is "smooth" when every radix is one of the unrolled ones.
 inline bool mr_fft_smooth(int n)
@@ -150,7 +155,7 @@ inline bool mr_fft_smooth(int n)
     int ns = mr_fft_factor(n, r);
     if (ns < 0) return false;
     for (int i = 0; i < ns; ++i)
-        if (r[i] > 8 || r[i] == 6) return false;
+        if (!mr_fft_unrolled(r[i])) return false;
     return true;
 }
 
@@ -269,11 +274,50 @@ template <int RR, bool INV, typename V> __device__ __forceinline__ void bfly_odd
     for (int q = 0; q < RR; ++q) v[q] = out[q];
 }
 
+template <int RR, bool INV, typename V> __device__ __forceinline__ void bfly(V *v);
+
+constexpr int mr_modinv(int a, int m)
+{
+    for (int x = 1; x < m; ++x)
+        if ((a * x) % m == 1) return x;
+    return 1;
+}
+
+// Composite radix R = R1 R2 with coprime factors by the prime-factor
+// (Good-Thomas) algorithm: input n = (R2 n1 + R1 n2) mod R, output
+// k = (R2 <R2^-1>_R1 k1 + R1 <R1^-1>_R2 k2) mod R, so that
+// W_R^{nk} = W_R1^{n1 k1} W_R2^{n2 k2}: R2 DFTs of length R1, then R1 DFTs of
+// length R2, with no twiddles between them.
+template <int R1, int R2, bool INV, typename V> __device__ __forceinline__ void bfly_pfa(V *v)
+{
+    constexpr int R = R1 * R2;
+    constexpr int e1 = R2 * mr_modinv(R2 % R1, R1);
+    constexpr int e2 = R1 * mr_modinv(R1 % R2, R2);
+    V y[R2][R1];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) y[n2][n1] = v[(R2 * n1 + R1 * n2) % R];
+        bfly<R1, INV>(y[n2]);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+        V t[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) t[n2] = y[n2][k1];
+        bfly<R2, INV>(t);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) v[(e1 * k1 + e2 * k2) % R] = t[k2];
+    }
+}
+
 template <int RR, bool INV, typename V> __device__ __forceinline__ void bfly(V *v)
 {
     if constexpr (RR == 2) bfly2<INV>(v);
     else if constexpr (RR == 4) bfly4<INV>(v);
     else if constexpr (RR == 8) bfly8<INV>(v);
+    else if constexpr (RR == 6) bfly_pfa<2, 3, INV>(v);
+    else if constexpr (RR == 10) bfly_pfa<2, 5, INV>(v);
     else bfly_odd<RR, INV>(v);
 }
 
@@ -352,8 +396,10 @@ __device__ typename cvec<R>::T *mr_fft_lds(const mr_fft_len<R> &f, typename cvec
         case 3: mr_stage<3, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
         case 4: mr_stage<4, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
         case 5: mr_stage<5, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
+        case 6: mr_stage<6, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
         case 7: mr_stage<7, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
         case 8: mr_stage<8, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
+        case 10: mr_stage<10, INV, V>(A, B, tw, f.n, ns, tp, Ns, md, nd); break;
         default: mr_stage_any<INV, V>(A, B, f.tw, f.n, ns, tp, Ns, RR); break;
         }
         __syncthreads();
@@ -472,7 +518,8 @@ mr_fft_axis_kernel(mr_fft_len<R> f, Load ld, Store st, long long npencil, int lg
 // Every stage first reads all of a thread's butterfly inputs into registers,
 // then (after a barrier) writes the outputs back in place, so a workgroup
 // needs TX*n + n LDS values and can hold twice the pencils of the
-// ping-pong kernel. Requires TX*n <= MR_FFT_EMAX and radices in {2..8}.
+// ping-pong kernel. Requires TX*n <= MR_FFT_EMAX and unrolled radices
+// (2..8, 10).
 // ---------------------------------------------------------------------------
 
 #define MR_FFT_EMAX 6144
@@ -556,7 +603,9 @@ mr_fft_axis_fast(mr_fft_len<R> f, Load ld, Store st, long long npencil, int lgTX
             case 3: mr_stage_reg<3, INV, V>(A, tw, n, ns, tp, Ns, md, nd); break;
             case 4: mr_stage_reg<4, INV, V>(A, tw, n, ns, tp, Ns, md, nd); break;
             case 5: mr_stage_reg<5, INV, V>(A, tw, n, ns, tp, Ns, md, nd); break;
+            case 6: mr_stage_reg<6, INV, V>(A, tw, n, ns, tp, Ns, md, nd); break;
             case 7: mr_stage_reg<7, INV, V>(A, tw, n, ns, tp, Ns, md, nd); break;
+            case 10: mr_stage_reg<10, INV, V>(A, tw, n, ns, tp, Ns, md, nd); break;
             default: mr_stage_reg<8, INV, V>(A, tw, n, ns, tp, Ns, md, nd); break;
             }
             Ns *= RR;

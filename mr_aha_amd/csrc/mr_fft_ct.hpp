@@ -8,7 +8,7 @@
 
 #include "mr_hip.hpp"
 
-// Stage radices of n by the rule of mr_fft_factor (8s, then 4, 2, 3, 5, 7)
+// Stage radices of n by the rule of mr_fft_factor (8s, then 10, 6, 4, 2, 3, 5, 7)
 // and the product Ns of the radices before each stage.
 struct mr_ct_fac {
     int nst;
@@ -20,9 +20,9 @@ struct mr_ct_fac {
 constexpr mr_ct_fac mr_ct_factor(int n)
 {
     mr_ct_fac f{};
-    const int small[6] = {8, 4, 2, 3, 5, 7};
+    const int small[8] = {8, 10, 6, 4, 2, 3, 5, 7};
     int m = n, Ns = 1;
-    for (int i = 0; i < 6; ++i)
+    for (int i = 0; i < 8; ++i)
         while (m % small[i] == 0 && f.nst < 16) {
             f.r[f.nst] = small[i];
             f.ns[f.nst] = Ns;

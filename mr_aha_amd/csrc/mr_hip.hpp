@@ -42,7 +42,7 @@ int mr_fft_pass(const mr_fft_len<R> &f, const Load &ld, const Store &st, long lo
     if (want_tx > MR_FFT_MAXTX) want_tx = MR_FFT_MAXTX;
     static const bool fast_on = mr_env_ll("MR_AHA_HIP_FFT_FAST", 1) != 0;
     bool smooth = sizeof(R) == 4 && fast_on;
-    for (int i = 0; i < f.nst && smooth; ++i) smooth = (f.radix[i] <= 8 && f.radix[i] != 6);
+    for (int i = 0; i < f.nst && smooth; ++i) smooth = mr_fft_unrolled(f.radix[i]);
     int TX;
     size_t lds;
     if (smooth && f.n <= MR_FFT_EMAX) {
