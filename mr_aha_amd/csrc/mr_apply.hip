@@ -1002,7 +1002,9 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
         mr_zmap zm{L, lgT};
         mr_ld_z_coil ld{A, zm, plane, Mo, L, nc};
         mr_st_z_conj_sum st{out, S, zm, plane, N, No, Mo, cno, coil0, coil0 > 0 ? 1 : 0};
-        MR_TRY((mr_fft_pass_ct<true>(fo, ld, st, plane * L, (size_t)No, 1 << lgT, s, "Io")));
+        // 32 pencils: the coil-sum accumulator makes 64 cost two workgroups per CU
+        MR_TRY((mr_fft_pass_ct<true>(fo, ld, st, plane * L, (size_t)No, 1 << (lgT < 5 ? lgT : 5),
+                                     s, "Io")));
         MR_TRY(mr_prof_mark(p, two ? MR_ST_I2 : MR_ST_I3));
     }
     return MR_SUCCESS;
