@@ -212,88 +212,6 @@ mr_fft_axis_ct(const float2 *tws, Load ld, Store st, long long npencil, int lgTX
     st.flush(p0, tp, acc);
 }
 
-// Pipelined variant for strided pencils (both functors with kElemFast false,
-// the y and outer-axis passes): a workgroup takes TPW consecutive tiles, each
-// with nsub rounds, and issues the global loads of the next round into
-// registers before it transforms the current one, so the HBM latency of a
-// round is hidden behind the FFT of the previous one. Pencil bases are double
-// buffered by round parity.
-template <int N, bool INV, class Load, class Store>
-__global__ void __launch_bounds__(MR_FFT_FAST_THREADS)
-mr_fft_axis_ct_pipe(const float2 *tws, Load ld, Store st, long long npencil, int lgTX, int tpw)
-{
-    constexpr int NT = MR_FFT_FAST_THREADS;
-    constexpr int NSTR = (N % 2 == 0) ? N + 1 : N;
-    constexpr int TPMAX = mr_ct_axis_tpmax<N>();
-    constexpr int KL = (N * TPMAX + NT - 1) / NT;
-    extern __shared__ __align__(16) unsigned char mr_smem[];
-    __shared__ typename Load::PB lpb[2][MR_FFT_MAXTX];
-    __shared__ typename Store::PB spb[2][MR_FFT_MAXTX];
-    const int TX = 1 << lgTX;
-    float2 *twl = reinterpret_cast<float2 *>(mr_smem);
-    float2 *A = twl + N;
-    float2 *acc = A + (size_t)TX * NSTR;
-    const long long ntiles = (npencil + TX - 1) / TX;
-    const long long t0 = mr_xcd_block(blockIdx.x, gridDim.x) * tpw;
-    const int ntl = (int)((ntiles - t0 < tpw) ? (ntiles - t0) : tpw);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int nsub = ld.nsub();
-    const int S = ntl * nsub;
-    for (int i = threadIdx.x; i < N; i += NT) twl[i] = tws[i];
-    const mr_tw_lds tw{twl};
-    auto tile_tp = [&](int s) {
-        const long long p0 = (t0 + s / nsub) * TX;
-        return (int)((npencil - p0 < TX) ? (npencil - p0) : TX);
-    };
-    auto bases = [&](int s) {
-        const long long p0 = (t0 + s / nsub) * TX;
-        const int sub = s % nsub, tp = tile_tp(s);
-        if ((int)threadIdx.x < tp) {
-            lpb[s & 1][threadIdx.x] = ld.pencil(p0 + threadIdx.x, sub);
-            spb[s & 1][threadIdx.x] = st.pencil(p0 + threadIdx.x, sub);
-        }
-    };
-    float2 pre[KL];
-    auto fetch = [&](int s) {
-        const int tp = tile_tp(s);
-#pragma unroll
-        for (int k = 0; k < KL; ++k) {
-            const int q = threadIdx.x + k * NT;
-            const int e = q >> lgTX, p = q & (TX - 1);
-            float2 v = make_float2(0.f, 0.f);
-            if (q < (N << lgTX) && p < tp) v = ld.get(lpb[s & 1][p], e);
-            pre[k] = v;
-        }
-    };
-    if (S > 0) {
-        bases(0);
-        __syncthreads();
-        fetch(0);
-    }
-    for (int s = 0; s < S; ++s) {
-        const int tp = tile_tp(s), sub = s % nsub;
-#pragma unroll
-        for (int k = 0; k < KL; ++k) {
-            const int q = threadIdx.x + k * NT;
-            const int e = q >> lgTX, p = q & (TX - 1);
-            if (q < (N << lgTX) && p < tp) A[p * NSTR + e] = pre[k];
-        }
-        if (s + 1 < S) bases(s + 1);
-        __syncthreads();
-        if (s + 1 < S) fetch(s + 1);
-        mr_ct_fft<N, NSTR, 0, INV, NT, TPMAX>(A, tw, tp, wave);
-        for (int q = threadIdx.x; q < (N << lgTX); q += NT) {
-            const int e = q >> lgTX, p = q & (TX - 1);
-            if (p < tp) st.put(spb[s & 1][p], p, e, sub, A[p * NSTR + e], acc);
-        }
-        __syncthreads();
-        if (sub == nsub - 1) {
-            st.flush((t0 + s / nsub) * TX, tp, acc);
-            __syncthreads();
-        }
-    }
-}
-
 // Host: launch the compile-time kernel for length N (tile choice as in
 // mr_fft_pass).
 template <int N, bool INV, class Load, class Store>
@@ -314,18 +232,6 @@ int mr_fft_pass_ct_n(const mr_fft_len<float> &f, const Load &ld, const Store &st
     const long long blocks = (npencil + TX - 1) / TX;
     if (blocks > 0x7fffffffLL)
         MR_FAIL(MR_ERR_UNSUPPORTED, "%s: %lld pencils exceed the launch limit", name, npencil);
-    if constexpr (!Load::kElemFast && !Store::kElemFast) {
-        // tiles per workgroup of the pipelined kernel (MR_AHA_HIP_FFT_TPW, 1 = plain)
-        const int tpw = (int)mr_env_ll("MR_AHA_HIP_FFT_TPW", 4);
-        if (tpw > 1) {
-            const long long wgs = (blocks + tpw - 1) / tpw;
-            hipLaunchKernelGGL((mr_fft_axis_ct_pipe<N, INV, Load, Store>), dim3((unsigned)wgs),
-                               dim3(MR_FFT_FAST_THREADS), need(TX), stream, f.tws, ld, st, npencil,
-                               lg, tpw);
-            MR_LAUNCH_CHECK(name);
-            return MR_SUCCESS;
-        }
-    }
     hipLaunchKernelGGL((mr_fft_axis_ct<N, INV, Load, Store>), dim3((unsigned)blocks),
                        dim3(MR_FFT_FAST_THREADS), need(TX), stream, f.tws, ld, st, npencil, lg);
     MR_LAUNCH_CHECK(name);
