@@ -960,7 +960,10 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
         mr_zmap zm{V, lgT};
         mr_ld_sx_z ld{x, S, zm, plane, N, No, Mo, cno, L, coil0};
         mr_st_z st{A, zm, plane, Mo};
-        MR_TRY((mr_fft_pass_ct<false>(fo, ld, st, plane * V, 0, 1 << lgT, s, "Fo")));
+        // pencils per workgroup: 2 measured fastest for 56-point z at 320x320
+        // (MR_AHA_HIP_FO_TX overrides; likewise MR_AHA_HIP_IO_TX below)
+        const int txo = (int)mr_env_ll("MR_AHA_HIP_FO_TX", 2);
+        MR_TRY((mr_fft_pass_ct<false>(fo, ld, st, plane * V, 0, txo, s, "Fo")));
         MR_TRY(mr_prof_mark(p, two ? MR_ST_F2 : MR_ST_F3));
     }
     if (!two) {
@@ -1003,8 +1006,8 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
         mr_ld_z_coil ld{A, zm, plane, Mo, L, nc};
         mr_st_z_conj_sum st{out, S, zm, plane, N, No, Mo, cno, coil0, coil0 > 0 ? 1 : 0};
         // 32 pencils: the coil-sum accumulator makes 64 cost two workgroups per CU
-        MR_TRY((mr_fft_pass_ct<true>(fo, ld, st, plane * L, (size_t)No, 1 << (lgT < 5 ? lgT : 5),
-                                     s, "Io")));
+        const int txi = (int)mr_env_ll("MR_AHA_HIP_IO_TX", 1 << (lgT < 5 ? lgT : 5));
+        MR_TRY((mr_fft_pass_ct<true>(fo, ld, st, plane * L, (size_t)No, txi, s, "Io")));
         MR_TRY(mr_prof_mark(p, two ? MR_ST_I2 : MR_ST_I3));
     }
     return MR_SUCCESS;

@@ -222,6 +222,8 @@ int mr_fft_pass_ct_n(const mr_fft_len<float> &f, const Load &ld, const Store &st
     constexpr int NSTR = (N % 2 == 0) ? N + 1 : N;
     if (npencil <= 0) return MR_SUCCESS;
     int TX = want_tx > MR_FFT_MAXTX ? MR_FFT_MAXTX : want_tx;
+    const long long txmax = mr_env_ll("MR_AHA_HIP_FFT_TXMAX", MR_FFT_MAXTX);
+    if (TX > txmax) TX = (int)(txmax < 1 ? 1 : txmax);
     auto need = [&](int tx) {
         return ((size_t)N + (size_t)tx * NSTR + (size_t)tx * acc_per_pencil) * sizeof(float2);
     };
