@@ -69,7 +69,12 @@ template <int D> constexpr bool mr_cdiv_exact()
 
 // Twiddle sources: a buffer resource over the global table, or the table
 // copied into LDS.
+// Twiddles from global memory (buffer loads): a butterfly loads only
+// w = W^k and forms W^(r k) = w^r by complex products (kPowers), since the
+// loads, not the arithmetic, are the scarce resource of the fused x pass.
+// Each power is a product of two lower ones, at most four roundings deep.
 struct mr_tw_buf {
+    static constexpr bool kPowers = true;
     __amdgpu_buffer_rsrc_t r;
     __device__ __forceinline__ float2 get(unsigned idx, unsigned off) const
     {
@@ -79,6 +84,7 @@ struct mr_tw_buf {
     }
 };
 struct mr_tw_lds {
+    static constexpr bool kPowers = false;
     const float2 *t;
     __device__ __forceinline__ float2 get(unsigned idx, unsigned off) const { return t[idx + off]; }
 };
@@ -110,8 +116,13 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
             for (int r = 0; r < R; ++r) v[k][r] = a[r * m];
             if constexpr (NS > 1) {
                 const unsigned kk = j - mr_cdiv<NS>::q(j) * NS;
+                if constexpr (TW::kPowers) {
+                    w[k][0] = tw.get(NS + kk - 1, 0u);
+                } else {
 #pragma unroll
-                for (int r = 1; r < R; ++r) w[k][r - 1] = tw.get(NS + kk - 1, (unsigned)((r - 1) * NS));
+                    for (int r = 1; r < R; ++r)
+                        w[k][r - 1] = tw.get(NS + kk - 1, (unsigned)((r - 1) * NS));
+                }
             }
         }
     }
@@ -123,6 +134,10 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
             const unsigned p = mr_cdiv<m>::q(f), j = f - p * m;
             const unsigned jn = mr_cdiv<NS>::q(j), kk = j - jn * NS;
             if constexpr (NS > 1) {
+                if constexpr (TW::kPowers) {
+#pragma unroll
+                    for (int r = 2; r < R; ++r) w[k][r - 1] = cmul(w[k][r / 2 - 1], w[k][r - r / 2 - 1]);
+                }
 #pragma unroll
                 for (int r = 1; r < R; ++r)
                     v[k][r] = INV ? cmulc(w[k][r - 1], v[k][r]) : cmul(w[k][r - 1], v[k][r]);

@@ -28,6 +28,36 @@ constexpr int mr_xbd_ct_gmax(int N, int L)
     return g < 1 ? 1 : g;
 }
 
+// Load the image points of one pencil into its N-point LDS row: source s
+// lands at e = (s - cnx) mod N. Chunks of 64 points, NC of them, issued back
+// to back; lanes past the Nx points (last chunk) read zero onto padding slots.
+template <int N, int NC>
+__device__ __forceinline__ void mr_xbd_load_chunks(float2 *Ap, __amdgpu_buffer_rsrc_t rs, int lane,
+                                                   int cnx)
+{
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int s = c * 64 + lane;
+        int e = s - cnx;
+        if (e < 0) e += N;
+        Ap[e] = mr_bload(rs, (unsigned)s * 8u);
+    }
+}
+
+// mr_xbd_load_chunks for nc = ceil(Nx / 64) chunks (wavefront-uniform).
+template <int N, int NC = N / 64>
+__device__ __forceinline__ void mr_xbd_load_src(float2 *Ap, __amdgpu_buffer_rsrc_t rs, int lane,
+                                                int cnx, int nc)
+{
+    if constexpr (NC >= 1) {
+        if (nc == NC) {
+            mr_xbd_load_chunks<N, NC>(Ap, rs, lane, cnx);
+            return;
+        }
+        mr_xbd_load_src<N, NC - 1>(Ap, rs, lane, cnx, nc);
+    }
+}
+
 template <int N, int L, bool MASKED, int KF>
 __global__ void __launch_bounds__(MR_XBD_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
 mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__restrict__ inv,
@@ -53,17 +83,19 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
     const long long pstride = (long long)Nx * R;
     const unsigned rowbytes = (unsigned)Nx * 8u;
 
+    // Only the Nx image points of a pencil are loaded (padding costs no load
+    // instructions, which are what bounds this kernel); the N - Nx padding
+    // slots e = (s - cnx) mod N, s >= Nx, are written with zeros.
+    const int nck = (Nx + 63) >> 6;
     for (int pl = wave; pl < tp; pl += NW) {
         const __amdgpu_buffer_rsrc_t rs = mr_rsrc(Yr + pstride * pl, rowbytes);
         float2 *Ap = A + pl * N;
-#pragma unroll
-        for (int e0 = 0; e0 < N; e0 += 64) {
-            const int e = e0 + lane;
-            if (e < N) {
-                int s = e + cnx;
-                if (s >= N) s -= N;
-                Ap[e] = mr_bload(rs, (unsigned)s * 8u);  // s >= Nx reads zero: the padding
-            }
+        mr_xbd_load_src<N>(Ap, rs, lane, cnx, nck);
+        for (int i = lane; i < N - Nx; i += 64) {
+            int e = Nx + i - cnx;
+            if (e >= N) e -= N;
+            if (e < 0) e += N;
+            Ap[e] = make_float2(0.f, 0.f);
         }
     }
     __syncthreads();
