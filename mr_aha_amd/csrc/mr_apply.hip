@@ -616,6 +616,7 @@ struct mr_aha_plan_s {
     size_t xbd_lds;     // its dynamic LDS bytes
     mr_fdiv nxdiv;
     mr_xbd_ct_fn xbd_ct;  // kernel specialised for (Mx, L), or null: mr_k_xbd
+    int xbd_nt;           // its workgroup size
     int kf;               // kernel format used by execute (MR_KF_*)
     const unsigned *kq;   // 16-bit entries and block scales (kf != FP32)
     const float *ksc;
@@ -722,6 +723,7 @@ static int mr_xbd_setup(mr_aha_plan p)
     p->xbd_g = 0;
     p->xbd_lds = 0;
     p->xbd_ct = nullptr;
+    p->xbd_nt = MR_XBD_THREADS;
     const mr_config &cfg = p->cfg;
     const int L = cfg.L, Mx = p->dims.m[0];
     if (mr_env_ll("MR_AHA_HIP_XBD", 1) == 0 || L > 16) return MR_SUCCESS;
@@ -760,6 +762,19 @@ static int mr_xbd_setup(mr_aha_plan p)
     g = mr_env_ll("MR_AHA_HIP_XBD_G", g);
     if (g > gmax) g = gmax;
     if (g < 1) g = 1;
+    int nt = MR_XBD_THREADS;
+    // Wide form (FP32 kernel, one coil per 80 KB): 1024 threads walk two
+    // coils per workgroup and load each kernel entry once for both
+    if (ct && g == 1 && p->kf == MR_KF_FP32 && cfg.n_coils >= 2 && 2 * per <= (size_t)maxlds &&
+        mr_env_ll("MR_AHA_HIP_XBD_WIDE", 1) != 0) {
+        mr_xbd_ct_fn wide = mr_xbd_ct_pick_wide(Mx, L, cfg.is_masked != 0);
+        if (wide) {
+            ct = wide;
+            fn = (const void *)wide;
+            g = 2;
+            nt = 1024;
+        }
+    }
     const size_t lds = per * (size_t)g;
     if (lds > 65536 &&
         hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
@@ -770,6 +785,7 @@ static int mr_xbd_setup(mr_aha_plan p)
     p->xbd_g = (int)g;
     p->xbd_lds = lds;
     p->xbd_ct = ct;
+    p->xbd_nt = nt;
     p->nxdiv = mr_fdiv_make((unsigned)p->dims.n[0]);
     return MR_SUCCESS;
 }
@@ -983,7 +999,7 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
             MR_FAIL(MR_ERR_UNSUPPORTED, "fused x pass: %lld workgroups exceed the launch limit",
                     blocks);
         if (p->xbd_ct) {
-            hipLaunchKernelGGL(p->xbd_ct, dim3((unsigned)blocks), dim3(MR_XBD_THREADS), p->xbd_lds,
+            hipLaunchKernelGGL(p->xbd_ct, dim3((unsigned)blocks), dim3((unsigned)p->xbd_nt), p->xbd_lds,
                                s, B, ker, p->d_inv, kstride, d, p->fx.tws, R, G, nc, ncg, p->kq,
                                p->ksc);
             MR_LAUNCH_CHECK("mr_k_xbd_ct");

@@ -102,6 +102,70 @@ __device__ __forceinline__ void mr_bd_walk_g(float2 *a, int ms, const float2 *ke
 }
 
 
+// Block multiply of one location for two coils at once (inputs a0[l*ms] and
+// a1[l*ms], in place), so that each packed kernel entry is loaded once for
+// both: the vector-memory instructions of the kernel stream, which bound the
+// fused pass, are halved per coil. Walk order and in-place semantics as
+// mr_bd_walk_g. The entries stream through a ring of D registers (entry
+// w + D is requested as entry w is consumed) so the two coils' outputs fit
+// in registers beside them; (pi, pj) is the walk index and row of the next
+// entry to request, advanced as the loop is unrolled, so every offset is a
+// constant.
+template <int L, bool UPPER, int D>
+__device__ __forceinline__ void mr_bd_walk_pair(float2 *a0, float2 *a1, int ms, const float2 *ker,
+                                                long long ks, unsigned boff)
+{
+    constexpr int T = L * (L + 1) / 2;
+    float2 y0[L], y1[L], kk[D];
+    int pi = 0, pj = 0;
+    auto ld = [&]() {
+        const int c = UPPER ? L - 1 - pi : pi;
+        const int t0 = UPPER ? c * (c + 1) / 2 : c * L - c * (c - 1) / 2;
+        const __amdgpu_buffer_rsrc_t rs = mr_rsrc(ker + ks * t0, 0xFFFFFFFFu);
+        const float2 e = mr_bload(rs, boff, (unsigned)(ks * 8 * pj));
+        if (++pj == L - pi) {
+            ++pi;
+            pj = 0;
+        }
+        return e;
+    };
+#pragma unroll
+    for (int w = 0; w < D; ++w)
+        if (w < T) kk[w] = ld();
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        y0[l] = make_float2(0.f, 0.f);
+        y1[l] = make_float2(0.f, 0.f);
+    }
+    int w = 0;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        const int c = UPPER ? L - 1 - i : i;
+        const float2 vc0 = a0[c * ms], vc1 = a1[c * ms];
+#pragma unroll
+        for (int j = 0; j < L - i; ++j, ++w) {
+            const float2 e = kk[w % D];
+            if (w + D < T) kk[w % D] = ld();
+            const int r = (UPPER ? 0 : c) + j;
+            if (r == c) {
+                y0[c] = cmac(y0[c], e, vc0);
+                y1[c] = cmac(y1[c], e, vc1);
+            } else {
+                y0[c] = cmac(y0[c], e, a0[r * ms]);
+                y0[r] = cmacc(y0[r], e, vc0);
+                y1[c] = cmac(y1[c], e, a1[r * ms]);
+                y1[r] = cmacc(y1[r], e, vc1);
+            }
+            // entry w is consumed before the next request: at most D entries
+            // are held in registers
+            asm volatile("" : "+v"(y0[c].x), "+v"(y0[c].y), "+v"(y0[r].x), "+v"(y0[r].y),
+                         "+v"(y1[c].x), "+v"(y1[c].y), "+v"(y1[r].x), "+v"(y1[r].y)::"memory");
+        }
+        a0[c * ms] = y0[c];
+        a1[c * ms] = y1[c];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 16-bit kernel format (BF16 or FP16 entries with FP32 scales per 4x4 block)
 // ---------------------------------------------------------------------------
@@ -229,6 +293,9 @@ typedef void (*mr_xbd_ct_fn)(float2 *, const float2 *, const int *, long long, m
                              const float2 *, long long, int, int, int, const unsigned *,
                              const float *);
 mr_xbd_ct_fn mr_xbd_ct_pick(int n, int L, bool masked, int kf, int *gmax);
+// The workgroup of 1024 threads and two coils (160 KB of LDS, one workgroup
+// per CU) for x length n and rank L with the FP32 kernel, or nullptr.
+mr_xbd_ct_fn mr_xbd_ct_pick_wide(int n, int L, bool masked);
 
 // Fill the 16-bit format q [T, kcol], sc [NB, kcol] from an FP32 kernel.
 int mr_kf_quantize(int kf, const float2 *ker, long long kcol, int L, int upper, unsigned *q,

@@ -58,14 +58,18 @@ __device__ __forceinline__ void mr_xbd_load_src(float2 *Ap, __amdgpu_buffer_rsrc
     }
 }
 
-template <int N, int L, bool MASKED, int KF>
-__global__ void __launch_bounds__(MR_XBD_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+// NT threads, up to GMAX coils per workgroup (the default: two workgroups of
+// 512 threads per CU; the wide form: 1024 threads and two coils in 160 KB,
+// one workgroup per CU). Either way a CU holds 16 wavefronts.
+template <int N, int L, bool MASKED, int KF, int NT = MR_XBD_THREADS,
+          int GMAX = mr_xbd_ct_gmax(N, L)>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
 mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__restrict__ inv,
             long long kstride, mr_dims d, const float2 *tws, long long R, int G, int ncoil, int ncg,
             const unsigned *__restrict__ kq, const float *__restrict__ ksc)
 {
-    constexpr int NW = MR_XBD_THREADS / 64;
-    constexpr int TPMAX = mr_xbd_ct_gmax(N, L) * L;
+    constexpr int NW = NT / 64;
+    constexpr int TPMAX = GMAX * L;
     extern __shared__ __align__(16) unsigned char mr_smem[];
     float2 *A = reinterpret_cast<float2 *>(mr_smem);
     const int Nx = d.n[0], cnx = d.c_n[0];
@@ -99,23 +103,54 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
         }
     }
     __syncthreads();
-    mr_ct_fft<N, N, 0, false, MR_XBD_THREADS, TPMAX>(A, twb, tp, wave);
+    mr_ct_fft<N, N, 0, false, NT, TPMAX>(A, twb, tp, wave);
 
     const int My = d.m[1];
     const int jy = (int)(r % My), jz = (int)(r / My);
     int ky = jy + d.c_m[1]; if (ky >= My) ky -= My;
     int kz = jz + d.c_m[2]; if (kz >= d.m[2]) kz -= d.m[2];
     const long long rowc = (long long)N * (ky + (long long)My * kz);
-    for (int wi = threadIdx.x; wi < gn * N; wi += MR_XBD_THREADS) {
-        const int g = (int)mr_cdiv<N>::q((unsigned)wi), e = wi - g * N;
-        long long b;
-        if (MASKED) {
-            b = inv[e + (long long)N * r];
-        } else {
-            int kx = e + d.c_m[0];
-            if (kx >= N) kx -= N;
-            b = rowc + kx;
+    // kernel index of location e of the row (-1 outside the mask)
+    auto kidx = [&](int e) -> long long {
+        if (MASKED) return inv[e + (long long)N * r];
+        int kx = e + d.c_m[0];
+        if (kx >= N) kx -= N;
+        return rowc + kx;
+    };
+    if constexpr (KF == MR_KF_FP32 && GMAX >= 2) {
+        // coils in pairs: a thread walks one location for two coils, loading
+        // each kernel entry once for both; an odd last coil walks alone
+        const int npair = gn >> 1;
+        for (int wi = threadIdx.x; wi < ((gn + 1) >> 1) * N; wi += NT) {
+            const int q = (int)mr_cdiv<N>::q((unsigned)wi), e = wi - q * N;
+            const long long b = kidx(e);
+            float2 *a0 = A + 2 * q * L * N + e;
+            if (q < npair) {
+                float2 *a1 = a0 + L * N;
+                if (b < 0) {
+#pragma unroll
+                    for (int l = 0; l < L; ++l) {
+                        a0[l * N] = make_float2(0.f, 0.f);
+                        a1[l * N] = make_float2(0.f, 0.f);
+                    }
+                } else if (d.tri_upper) {
+                    mr_bd_walk_pair<L, true, (L > 8 ? 8 : 16)>(a0, a1, N, ker, kstride, (unsigned)(b * 8));
+                } else {
+                    mr_bd_walk_pair<L, false, (L > 8 ? 8 : 16)>(a0, a1, N, ker, kstride, (unsigned)(b * 8));
+                }
+            } else if (b < 0) {
+#pragma unroll
+                for (int l = 0; l < L; ++l) a0[l * N] = make_float2(0.f, 0.f);
+            } else if (d.tri_upper) {
+                mr_bd_walk_g<L, true>(a0, N, ker, kstride, (unsigned)(b * 8));
+            } else {
+                mr_bd_walk_g<L, false>(a0, N, ker, kstride, (unsigned)(b * 8));
+            }
         }
+    } else {
+    for (int wi = threadIdx.x; wi < gn * N; wi += NT) {
+        const int g = (int)mr_cdiv<N>::q((unsigned)wi), e = wi - g * N;
+        const long long b = kidx(e);
         float2 *a = A + g * L * N + e;
         if (b < 0) {
 #pragma unroll
@@ -131,8 +166,9 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
             mr_bd_walk_g<L, false>(a, N, ker, kstride, (unsigned)(b * 8));
         }
     }
+    }
     __syncthreads();
-    mr_ct_fft<N, N, 0, true, MR_XBD_THREADS, TPMAX>(A, twb, tp, wave);
+    mr_ct_fft<N, N, 0, true, NT, TPMAX>(A, twb, tp, wave);
 
     for (int pl = wave; pl < tp; pl += NW) {
         const __amdgpu_buffer_rsrc_t rs = mr_rsrc(Yr + pstride * pl, rowbytes);
@@ -220,6 +256,17 @@ __global__ void mr_k_quantize(const float2 *__restrict__ ker, long long kcol, in
 }
 
 }  // namespace
+
+mr_xbd_ct_fn mr_xbd_ct_pick_wide(int n, int L, bool masked)
+{
+    if (n == 640 && L == 16)
+        return masked ? mr_k_xbd_ct<640, 16, true, MR_KF_FP32, 1024, 2>
+                      : mr_k_xbd_ct<640, 16, false, MR_KF_FP32, 1024, 2>;
+    if (n == 512 && L == 16)
+        return masked ? mr_k_xbd_ct<512, 16, true, MR_KF_FP32, 1024, 2>
+                      : mr_k_xbd_ct<512, 16, false, MR_KF_FP32, 1024, 2>;
+    return nullptr;
+}
 
 mr_xbd_ct_fn mr_xbd_ct_pick(int n, int L, bool masked, int kf, int *gmax)
 {

@@ -207,8 +207,10 @@ def test_fused_and_separate_routes(mr, monkeypatch, name, masked):
 SPECIALISED = [("2D", (320, 12), 16, 2), ("2D", (160, 10), 8, 3), ("2D", (128, 8), 4, 5),
                ("2D", (192, 6), 16, 2), ("2D", (256, 6), 8, 3), ("stack", (320, 8, 3), 16, 2),
                ("3D", (160, 6, 4), 4, 3), ("2D", (200, 6), 8, 2),
-               # last partial round of walks split over 2 and 4 wavefronts
-               ("2D", (320, 10), 8, 2), ("2D", (192, 8), 8, 3)]
+               ("2D", (320, 10), 8, 2), ("2D", (192, 8), 8, 3),
+               # coil pairs sharing kernel loads, with an odd last coil: x length
+               # 320 (two coils in 80 KB) and 512 (1024-thread workgroups, 160 KB)
+               ("2D", (160, 8), 16, 3), ("2D", (256, 8), 16, 3)]
 
 
 @pytest.mark.parametrize("tt,n,L,C", SPECIALISED)
