@@ -763,11 +763,11 @@ static int mr_xbd_setup(mr_aha_plan p)
     if (g > gmax) g = gmax;
     if (g < 1) g = 1;
     int nt = MR_XBD_THREADS;
-    // Wide form (FP32 kernel, one coil per 80 KB): 1024 threads walk two
-    // coils per workgroup and load each kernel entry once for both
-    if (ct && g == 1 && p->kf == MR_KF_FP32 && cfg.n_coils >= 2 && 2 * per <= (size_t)maxlds &&
+    // Wide form (one coil per 80 KB): 1024 threads walk two coils per
+    // workgroup and load each kernel entry once for both
+    if (ct && g == 1 && cfg.n_coils >= 2 && 2 * per <= (size_t)maxlds &&
         mr_env_ll("MR_AHA_HIP_XBD_WIDE", 1) != 0) {
-        mr_xbd_ct_fn wide = mr_xbd_ct_pick_wide(Mx, L, cfg.is_masked != 0);
+        mr_xbd_ct_fn wide = mr_xbd_ct_pick_wide(Mx, L, cfg.is_masked != 0, p->kf);
         if (wide) {
             ct = wide;
             fn = (const void *)wide;

@@ -105,7 +105,7 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
     static_assert(mr_cdiv_exact<m>() && mr_cdiv_exact<NS>(), "inexact constant division");
     const int nb = tp * m;
     float2 v[KMAX][R];
-    float2 w[KMAX][RW];
+    float2 w[KMAX][TW::kPowers ? 1 : RW];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         if (k * NT + wave * 64 < nb) {
@@ -121,7 +121,7 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                 } else {
 #pragma unroll
                     for (int r = 1; r < R; ++r)
-                        w[k][r - 1] = tw.get(NS + kk - 1, (unsigned)((r - 1) * NS));
+                        w[k][TW::kPowers ? 0 : r - 1] = tw.get(NS + kk - 1, (unsigned)((r - 1) * NS));
                 }
             }
         }
@@ -133,14 +133,20 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
             const unsigned f = min((unsigned)(threadIdx.x + k * NT), (unsigned)(nb - 1));
             const unsigned p = mr_cdiv<m>::q(f), j = f - p * m;
             const unsigned jn = mr_cdiv<NS>::q(j), kk = j - jn * NS;
-            if constexpr (NS > 1) {
-                if constexpr (TW::kPowers) {
+            if constexpr (NS > 1 && TW::kPowers) {
+                // each power applied as soon as it is formed, so few are live
+                float2 wp[RW];
+                wp[0] = w[k][0];
 #pragma unroll
-                    for (int r = 2; r < R; ++r) w[k][r - 1] = cmul(w[k][r / 2 - 1], w[k][r - r / 2 - 1]);
+                for (int r = 1; r < R; ++r) {
+                    if (r >= 2) wp[r - 1] = cmul(wp[r / 2 - 1], wp[r - r / 2 - 1]);
+                    v[k][r] = INV ? cmulc(wp[r - 1], v[k][r]) : cmul(wp[r - 1], v[k][r]);
                 }
+            } else if constexpr (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r)
-                    v[k][r] = INV ? cmulc(w[k][r - 1], v[k][r]) : cmul(w[k][r - 1], v[k][r]);
+                    v[k][r] = INV ? cmulc(w[k][TW::kPowers ? 0 : r - 1], v[k][r])
+                                  : cmul(w[k][TW::kPowers ? 0 : r - 1], v[k][r]);
             }
             bfly<R, INV>(v[k]);
             float2 *b = A + p * NSTR + jn * (NS * R) + kk;

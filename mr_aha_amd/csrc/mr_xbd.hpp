@@ -294,8 +294,8 @@ typedef void (*mr_xbd_ct_fn)(float2 *, const float2 *, const int *, long long, m
                              const float *);
 mr_xbd_ct_fn mr_xbd_ct_pick(int n, int L, bool masked, int kf, int *gmax);
 // The workgroup of 1024 threads and two coils (160 KB of LDS, one workgroup
-// per CU) for x length n and rank L with the FP32 kernel, or nullptr.
-mr_xbd_ct_fn mr_xbd_ct_pick_wide(int n, int L, bool masked);
+// per CU) for x length n, rank L and kernel format kf, or nullptr.
+mr_xbd_ct_fn mr_xbd_ct_pick_wide(int n, int L, bool masked, int kf);
 
 // Fill the 16-bit format q [T, kcol], sc [NB, kcol] from an FP32 kernel.
 int mr_kf_quantize(int kf, const float2 *ker, long long kcol, int L, int upper, unsigned *q,

@@ -119,9 +119,11 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
     };
     if constexpr (KF == MR_KF_FP32 && GMAX >= 2) {
         // coils in pairs: a thread walks one location for two coils, loading
-        // each kernel entry once for both; an odd last coil walks alone
+        // each kernel entry once for both; an odd last coil walks alone (the
+        // 16-bit kernel, half the bytes per entry, measured no gain from it)
         const int npair = gn >> 1;
-        for (int wi = threadIdx.x; wi < ((gn + 1) >> 1) * N; wi += NT) {
+        const int nitem = ((gn + 1) >> 1) * N;
+        auto item = [&](int wi) {
             const int q = (int)mr_cdiv<N>::q((unsigned)wi), e = wi - q * N;
             const long long b = kidx(e);
             float2 *a0 = A + 2 * q * L * N + e;
@@ -146,6 +148,13 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
             } else {
                 mr_bd_walk_g<L, false>(a0, N, ker, kstride, (unsigned)(b * 8));
             }
+        };
+        if constexpr (((GMAX + 1) / 2) * N <= NT) {
+            // a thread per item: one round, no loop (a loop here keeps the
+            // walk's column bases live across iterations and spills)
+            if ((int)threadIdx.x < nitem) item(threadIdx.x);
+        } else {
+            for (int wi = threadIdx.x; wi < nitem; wi += NT) item(wi);
         }
     } else {
     for (int wi = threadIdx.x; wi < gn * N; wi += NT) {
@@ -257,15 +266,16 @@ __global__ void mr_k_quantize(const float2 *__restrict__ ker, long long kcol, in
 
 }  // namespace
 
-mr_xbd_ct_fn mr_xbd_ct_pick_wide(int n, int L, bool masked)
+mr_xbd_ct_fn mr_xbd_ct_pick_wide(int n, int L, bool masked, int kf)
 {
-    if (n == 640 && L == 16)
+    // FP32 only: the 16-bit walk moves half the bytes and measured slower in
+    // the wide form (one workgroup per CU) than in two 80 KB workgroups
+    if (L != 16 || (n != 640 && n != 512) || kf != MR_KF_FP32) return nullptr;
+    if (n == 640)
         return masked ? mr_k_xbd_ct<640, 16, true, MR_KF_FP32, 1024, 2>
                       : mr_k_xbd_ct<640, 16, false, MR_KF_FP32, 1024, 2>;
-    if (n == 512 && L == 16)
-        return masked ? mr_k_xbd_ct<512, 16, true, MR_KF_FP32, 1024, 2>
-                      : mr_k_xbd_ct<512, 16, false, MR_KF_FP32, 1024, 2>;
-    return nullptr;
+    return masked ? mr_k_xbd_ct<512, 16, true, MR_KF_FP32, 1024, 2>
+                  : mr_k_xbd_ct<512, 16, false, MR_KF_FP32, 1024, 2>;
 }
 
 mr_xbd_ct_fn mr_xbd_ct_pick(int n, int L, bool masked, int kf, int *gmax)
