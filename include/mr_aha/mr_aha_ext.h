@@ -35,6 +35,29 @@ MR_EXPORT int mr_op_profile(mr_op op, double *ms, int *calls, int n);
 MR_EXPORT int mr_aha_plan_xbd_group(mr_aha_plan plan);
 MR_EXPORT int mr_op_xbd_group(mr_op op);
 
+/* Autotuner of the fused route (no counterpart in the reference, whose cuFFT
+ * plans fix their own tiling). For each axis pass of the apply (outer-axis
+ * FFT, y-FFT, inverse y-FFT, inverse outer-axis FFT; 2D: the first and last)
+ * the tile widths 1, 2, ..., 64 pencils per workgroup are timed on the
+ * caller's data, `reps` applies each (1..100), and the fastest is kept for
+ * later executes; then the two-coil form of the fused x pass is timed
+ * against one coil per workgroup. Results do not depend on the choice. The
+ * arrays are those of mr_aha_plan_execute / mr_op_execute; d_AhAx receives
+ * the last trial's output. Synchronous. Clears the per-stage profile. A plan
+ * on separate FFT passes is left unchanged. MR_AHA_HIP_{FO,FY,IY,IO}_TX
+ * environment settings still override the tuned widths. */
+MR_EXPORT int mr_aha_plan_autotune(mr_aha_plan plan, const mr_complex *d_x,
+                                   const mr_complex *d_SEs, const mr_complex *d_phi2_toep,
+                                   const int64_t *d_mask, mr_complex *d_AhAx, int reps);
+MR_EXPORT int mr_op_autotune(mr_op op, const mr_complex *d_x, mr_complex *d_AhAx, int reps);
+
+/* Tiling in use: tiles[0..3] = pencils per workgroup requested for the
+ * outer-axis, y, inverse y and inverse outer-axis passes (0: the built-in
+ * rule), tiles[4] = 1 when the fused x pass runs two-coil 1024-thread
+ * workgroups. Writes min(n, 5) entries. No counterpart in the reference. */
+MR_EXPORT int mr_aha_plan_tiles(mr_aha_plan plan, int *tiles, int n);
+MR_EXPORT int mr_op_tiles(mr_op op, int *tiles, int n);
+
 /* Conjugate gradients on (A^H A + lambda I) x = b with operator `op`
  * (FP32 vectors; FP64 inner products and step lengths, kept on the device).
  * Every `refresh` iterations (0 = never) the recursive residual is replaced

@@ -383,6 +383,37 @@ class AhAOp:
     def kernel_format(self) -> str:
         return getattr(self, "_kfmt", "fp32")
 
+    def autotune(self, x=None, reps: int = 2) -> dict:
+        """Time the tile widths of the apply's FFT passes and the fused x
+        pass's two-coil form on this operator and keep the fastest (library
+        entry mr_op_autotune; results do not depend on the choice). ``x`` is
+        the image the trials apply to (default: a seeded complex Gaussian of
+        the operator's shape). Returns the tiling now in use (`tiles`). No
+        counterpart in the reference."""
+        self._check_open()
+        nx = int(lib.mr_x_size(ctypes.byref(self._cfg)))
+        if x is None:
+            g = torch.Generator(device="cpu").manual_seed(1234)
+            x = torch.complex(torch.randn(nx, generator=g), torch.randn(nx, generator=g))
+            x = x.to(self._device)
+        _check_size(x, nx, "x")
+        out = torch.empty(x.shape, dtype=torch.complex64, device=x.device)
+        torch.cuda.synchronize(self._device)   # x written on torch's stream is complete
+        check(lib.mr_op_autotune(self._handle, _ptr(x, "x", "complex64"),
+                                 _ptr(out, "out", "complex64"), int(reps)), "mr_op_autotune")
+        return self.tiles
+
+    @property
+    def tiles(self) -> dict:
+        """Tiling in use (library entry mr_op_tiles): pencils per workgroup
+        requested for the outer-axis, y, inverse y and inverse outer-axis FFT
+        passes (0: the built-in rule) and whether the fused x pass runs
+        two-coil workgroups."""
+        self._check_open()
+        t = (ctypes.c_int * 5)()
+        check(lib.mr_op_tiles(self._handle, t, 5), "mr_op_tiles")
+        return dict(fo=t[0], fy=t[1], iy=t[2], io=t[3], wide=bool(t[4]))
+
     def cg(self, b, x=None, *, lam: float = 0.0, iters: int = 40, refresh: int = 10,
            op_refresh: Optional["AhAOp"] = None, graph: bool = False):
         """Conjugate gradients on (A^H A + lam I) x = b (library entry mr_op_cg).

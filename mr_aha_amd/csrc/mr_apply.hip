@@ -617,6 +617,9 @@ struct mr_aha_plan_s {
     mr_fdiv nxdiv;
     mr_xbd_ct_fn xbd_ct;  // kernel specialised for (Mx, L), or null: mr_k_xbd
     int xbd_nt;           // its workgroup size
+    int tile[4];          // pencils per workgroup of Fo, Fy, Iy, Io (0: default rule)
+    int no_wide;          // 1: no two-coil 1024-thread fused workgroups
+    bool tuning;          // mr_aha_plan_autotune is timing stages
     int kf;               // kernel format used by execute (MR_KF_*)
     const unsigned *kq;   // 16-bit entries and block scales (kf != FP32)
     const float *ksc;
@@ -765,7 +768,7 @@ static int mr_xbd_setup(mr_aha_plan p)
     int nt = MR_XBD_THREADS;
     // Wide form (one coil per 80 KB): 1024 threads walk two coils per
     // workgroup and load each kernel entry once for both
-    if (ct && g == 1 && cfg.n_coils >= 2 && 2 * per <= (size_t)maxlds &&
+    if (ct && g == 1 && cfg.n_coils >= 2 && 2 * per <= (size_t)maxlds && !p->no_wide &&
         mr_env_ll("MR_AHA_HIP_XBD_WIDE", 1) != 0) {
         mr_xbd_ct_fn wide = mr_xbd_ct_pick_wide(Mx, L, cfg.is_masked != 0, p->kf);
         if (wide) {
@@ -976,16 +979,18 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
         mr_zmap zm{V, lgT};
         mr_ld_sx_z ld{x, S, zm, plane, N, No, Mo, cno, L, coil0};
         mr_st_z st{A, zm, plane, Mo};
-        // pencils per workgroup: 2 measured fastest for 56-point z at 320x320
-        // (MR_AHA_HIP_FO_TX overrides; likewise MR_AHA_HIP_IO_TX below)
-        const int txo = (int)mr_env_ll("MR_AHA_HIP_FO_TX", 2);
+        // pencils per workgroup: the autotuned width, else 2 (measured fastest
+        // for 56-point z at 320x320); MR_AHA_HIP_FO_TX overrides both, and
+        // likewise for the other passes below
+        const int txo = (int)mr_env_ll("MR_AHA_HIP_FO_TX", p->tile[0] > 0 ? p->tile[0] : 2);
         MR_TRY((mr_fft_pass_ct<false>(fo, ld, st, plane * V, 0, txo, s, "Fo")));
         MR_TRY(mr_prof_mark(p, two ? MR_ST_F2 : MR_ST_F3));
     }
     if (!two) {
         mr_ld_pad<float2, false> ld{A, Nx, My, Ny, d.c_n[1]};
         mr_st_axis<float2, false> st{B, Nx, My};
-        MR_TRY((mr_fft_pass_ct<false>(p->fy, ld, st, (long long)Nx * Mz * V, 0, 16, s, "Fy")));
+        const int txy = (int)mr_env_ll("MR_AHA_HIP_FY_TX", p->tile[1] > 0 ? p->tile[1] : 16);
+        MR_TRY((mr_fft_pass_ct<false>(p->fy, ld, st, (long long)Nx * Mz * V, 0, txy, s, "Fy")));
         MR_TRY(mr_prof_mark(p, MR_ST_F2));
     }
     {
@@ -1014,7 +1019,8 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
     if (!two) {
         mr_ld_axis<float2, false> ld{B, Nx, My};
         mr_st_crop<float2, false> st{A, Nx, My, Ny, d.c_n[1]};
-        MR_TRY((mr_fft_pass_ct<true>(p->fy, ld, st, (long long)Nx * Mz * V, 0, 16, s, "Iy")));
+        const int txy = (int)mr_env_ll("MR_AHA_HIP_IY_TX", p->tile[2] > 0 ? p->tile[2] : 16);
+        MR_TRY((mr_fft_pass_ct<true>(p->fy, ld, st, (long long)Nx * Mz * V, 0, txy, s, "Iy")));
         MR_TRY(mr_prof_mark(p, MR_ST_I2));
     }
     {
@@ -1022,7 +1028,8 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
         mr_ld_z_coil ld{A, zm, plane, Mo, L, nc};
         mr_st_z_conj_sum st{out, S, zm, plane, N, No, Mo, cno, coil0, coil0 > 0 ? 1 : 0};
         // 32 pencils: the coil-sum accumulator makes 64 cost two workgroups per CU
-        const int txi = (int)mr_env_ll("MR_AHA_HIP_IO_TX", 1 << (lgT < 5 ? lgT : 5));
+        const int txi = (int)mr_env_ll("MR_AHA_HIP_IO_TX",
+                                       p->tile[3] > 0 ? p->tile[3] : 1 << (lgT < 5 ? lgT : 5));
         MR_TRY((mr_fft_pass_ct<true>(fo, ld, st, plane * L, (size_t)No, txi, s, "Io")));
         MR_TRY(mr_prof_mark(p, two ? MR_ST_I2 : MR_ST_I3));
     }
@@ -1041,7 +1048,7 @@ static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, cons
     const long long N = d.image_size;
     const long long NyNz = (long long)Ny * Nz;
     const bool three = cfg.n_spatial == 3;
-    p->prof_on = mr_env_ll("MR_AHA_HIP_PROFILE", 0) != 0;
+    p->prof_on = p->tuning || mr_env_ll("MR_AHA_HIP_PROFILE", 0) != 0;
     MR_TRY(mr_prof_mark(p, -1));
 
     if (cfg.is_masked) {
@@ -1187,6 +1194,104 @@ extern "C" MR_EXPORT int mr_aha_plan_execute(mr_aha_plan plan, const mr_complex 
         rc = mr_aha_plan_execute_raw(plan, d_x, d_SEs, d_phi2_toep, d_mask, d_AhAx);
     mr_pop_device(prev);
     return rc;
+}
+
+// Autotuner of the fused route: for each axis pass (Fo, Fy, Iy, Io; 2D: Fo,
+// Io) the tile widths 1..64 pencils are timed in turn on the caller's data
+// (`reps` applies each, HIP events per stage) and the fastest is kept; then
+// the fused x pass's two-coil wide form against one coil per workgroup.
+// Every candidate computes the same arithmetic per pencil, so results do not
+// depend on the choice. The per-stage profile totals are cleared.
+static int mr_autotune_impl(mr_aha_plan p, const mr_complex *x, const mr_complex *S,
+                            const mr_complex *ker, const int64_t *mask, mr_complex *out, int reps)
+{
+    if (p->xbd_g == 0) return MR_SUCCESS;  // separate passes: fixed tiling
+    const bool two = p->cfg.n_spatial == 2;
+    auto zero_prof = [&]() -> int {
+        MR_TRY(mr_prof_collect(p));
+        for (int i = 0; i < MR_N_ST; ++i) {
+            p->prof.ms[i] = 0.0;
+            p->prof.calls[i] = 0;
+        }
+        return MR_SUCCESS;
+    };
+    // mean ms of one stage over `reps` applies
+    auto stage_ms = [&](int stage, double *ms) -> int {
+        MR_TRY(zero_prof());
+        for (int r = 0; r < reps; ++r) MR_TRY(mr_aha_plan_execute_raw(p, x, S, ker, mask, out));
+        MR_TRY(mr_prof_collect(p));
+        *ms = p->prof.calls[stage] > 0 ? p->prof.ms[stage] / p->prof.calls[stage] : 0.0;
+        return MR_SUCCESS;
+    };
+    p->tuning = true;
+    int rc = mr_aha_plan_execute_raw(p, x, S, ker, mask, out);  // warm-up
+    const int idx[4] = {0, 1, 2, 3};
+    const int stg[4] = {two ? MR_ST_F2 : MR_ST_F3, MR_ST_F2, MR_ST_I2, two ? MR_ST_I2 : MR_ST_I3};
+    for (int k = 0; k < 4 && rc == MR_SUCCESS; ++k) {
+        const int t = idx[k];
+        if (two && (t == 1 || t == 2)) continue;  // 2D: no separate y passes
+        int best = p->tile[t];
+        double best_ms = 1e30;
+        for (int cand = 1; cand <= 64 && rc == MR_SUCCESS; cand *= 2) {
+            p->tile[t] = cand;
+            double ms = 0.0;
+            rc = stage_ms(stg[t], &ms);
+            if (rc == MR_SUCCESS && ms < best_ms) {
+                best_ms = ms;
+                best = cand;
+            }
+        }
+        p->tile[t] = best;
+    }
+    if (rc == MR_SUCCESS && p->xbd_nt > MR_XBD_THREADS) {
+        double wide_ms = 0.0, narrow_ms = 0.0;
+        rc = stage_ms(MR_ST_BD, &wide_ms);
+        if (rc == MR_SUCCESS) {
+            p->no_wide = 1;
+            rc = mr_xbd_setup(p);
+            if (rc == MR_SUCCESS) rc = stage_ms(MR_ST_BD, &narrow_ms);
+            if (rc == MR_SUCCESS && narrow_ms >= wide_ms) {
+                p->no_wide = 0;
+                rc = mr_xbd_setup(p);
+            }
+        }
+    }
+    p->tuning = false;
+    const int rc2 = zero_prof();
+    return rc != MR_SUCCESS ? rc : rc2;
+}
+
+extern "C" MR_EXPORT int mr_aha_plan_autotune(mr_aha_plan plan, const mr_complex *d_x,
+                                              const mr_complex *d_SEs,
+                                              const mr_complex *d_phi2_toep,
+                                              const int64_t *d_mask, mr_complex *d_AhAx, int reps)
+{
+    mr_clear_error();
+    MR_REQUIRE(plan != nullptr, "mr_aha_plan_autotune: plan is NULL");
+    MR_REQUIRE(d_x != nullptr && d_SEs != nullptr && d_phi2_toep != nullptr && d_AhAx != nullptr,
+               "mr_aha_plan_autotune: NULL array");
+    MR_REQUIRE((const void *)d_AhAx != (const void *)d_x,
+               "mr_aha_plan_autotune: d_AhAx must not alias d_x");
+    MR_REQUIRE(reps >= 1 && reps <= 100, "mr_aha_plan_autotune: reps %d is not in 1..100", reps);
+    if (plan->cfg.is_masked)
+        MR_REQUIRE(d_mask != nullptr,
+                   "mr_aha_plan_autotune: d_mask is required when is_masked is set");
+    int prev = -1;
+    int rc = mr_push_device(&plan->cfg, &prev);
+    if (rc == MR_SUCCESS) rc = mr_autotune_impl(plan, d_x, d_SEs, d_phi2_toep, d_mask, d_AhAx, reps);
+    mr_pop_device(prev);
+    return rc;
+}
+
+extern "C" MR_EXPORT int mr_aha_plan_tiles(mr_aha_plan plan, int *tiles, int n)
+{
+    mr_clear_error();
+    MR_REQUIRE(plan != nullptr, "mr_aha_plan_tiles: plan is NULL");
+    MR_REQUIRE(tiles != nullptr || n == 0, "mr_aha_plan_tiles: tiles is NULL");
+    const int v[5] = {plan->tile[0], plan->tile[1], plan->tile[2], plan->tile[3],
+                      plan->xbd_nt > MR_XBD_THREADS ? 1 : 0};
+    for (int i = 0; i < n && i < 5; ++i) tiles[i] = v[i];
+    return MR_SUCCESS;
 }
 
 extern "C" MR_EXPORT int mr_aha_toeplitz(const mr_config *cfg, const mr_complex *d_x,

@@ -436,6 +436,32 @@ extern "C" MR_EXPORT int mr_op_set_kernel_format(mr_op op, int format)
     return rc;
 }
 
+extern "C" MR_EXPORT int mr_op_autotune(mr_op op, const mr_complex *d_x, mr_complex *d_AhAx,
+                                        int reps)
+{
+    mr_clear_error();
+    MR_REQUIRE(op != nullptr, "mr_op_autotune: op is NULL");
+    if (!op->plan) return MR_SUCCESS;  // NUFFT operators have nothing to tune
+    int prev = 0;
+    MR_HIP_CHECK(hipGetDevice(&prev));
+    if (prev != op->primary) MR_HIP_CHECK(hipSetDevice(op->primary));
+    const int rc = mr_aha_plan_autotune(op->plan, d_x, op->d_SEs, op->d_toep, op->d_mask, d_AhAx,
+                                        reps);
+    if (prev != op->primary) (void)hipSetDevice(prev);
+    return rc;
+}
+
+extern "C" MR_EXPORT int mr_op_tiles(mr_op op, int *tiles, int n)
+{
+    mr_clear_error();
+    MR_REQUIRE(op != nullptr, "mr_op_tiles: op is NULL");
+    if (!op->plan) {
+        for (int i = 0; i < n && tiles; ++i) tiles[i] = 0;
+        return MR_SUCCESS;
+    }
+    return mr_aha_plan_tiles(op->plan, tiles, n);
+}
+
 extern "C" MR_EXPORT int mr_op_xbd_group(mr_op op)
 {
     mr_clear_error();

@@ -28,7 +28,7 @@ def test_exports_every_declared_symbol(L):
     syms = declared_symbols()
     # 25 reference entry points + mr_version + 6 extensions (profile, pipeline, CG,
     # kernel format)
-    assert len(syms) == 32
+    assert len(syms) == 36
     for s in syms:
         assert hasattr(L.lib, s), s
     out = subprocess.run(["nm", "-D", "--defined-only", str(L.LIB_PATH)], capture_output=True,

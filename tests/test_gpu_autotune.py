@@ -1,0 +1,102 @@
+"""GPU tests of the tile autotuner (mr_op_autotune / mr_aha_plan_autotune, no
+counterpart in the reference): it keeps a tiling from the candidates it times,
+the applies after it agree with the applies before it (the tile width changes
+only which pencils share a workgroup, not the arithmetic of any pencil, so
+equality is bit for bit), and its argument checks return the documented
+errors."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def mr():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import mr_aha_amd
+    return mr_aha_amd
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+# (traj type, n, L, C): stack and 3D run all four axis passes, 2D the outer
+# two; (320, 12) L16 C2 has an x grid of 640 and also times the two-coil form
+CASES = [("stack", (48, 40, 10), 4, 3), ("3D", (24, 20, 16), 4, 2), ("2D", (60, 44), 8, 3),
+         ("2D", (320, 12), 16, 2)]
+
+
+@pytest.mark.parametrize("tt,n,L,C", CASES)
+@pytest.mark.parametrize("masked", [False, True])
+def test_autotune_keeps_results(mr, tt, n, L, C, masked):
+    from oracle import conventions as cv
+    from oracle import toeplitz
+    print(f"[autotune] {tt} {n} L={L} C={C} masked={masked}", flush=True)
+    rng = np.random.default_rng(11 * L + C)
+    g = cv.make_geometry(tt, n, L, C)
+    mask = None
+    if masked:
+        mask = np.flatnonzero(rng.random(g.P) < 0.7).astype(np.int64)
+        kshape = (g.T, mask.size)
+    else:
+        kshape = (g.T,) + g.grid_shape()
+    K = (rng.standard_normal(kshape) + 1j * rng.standard_normal(kshape)).astype(np.complex64)
+    x = (rng.standard_normal((L,) + g.img_shape())
+         + 1j * rng.standard_normal((L,) + g.img_shape())).astype(np.complex64)
+    S = (rng.standard_normal((C,) + g.img_shape())
+         + 1j * rng.standard_normal((C,) + g.img_shape())).astype(np.complex64)
+    cfg = mr.make_config(g.traj_type, g.n[: g.n_spatial], L, phi2_dim=g.M[: g.n_spatial],
+                         n_coils=C, is_masked=masked,
+                         num_masked=int(mask.size) if masked else 0)
+    dx, dS, dK = dev(x), dev(S), dev(K)
+    dmask = dev(mask) if masked else None
+    ref = toeplitz.apply(g, x, S, K.astype(np.complex128), upper=False, mask=mask)
+    with mr.create_op(cfg, dS, None, None, _phi2_toep=dK, mask=dmask) as op:
+        y0 = host(op(dx))
+        assert rel(y0, ref) < 1e-5
+        before = op.tiles
+        assert (before["fo"], before["fy"], before["iy"], before["io"]) == (0, 0, 0, 0)
+        after = op.autotune(dx, reps=1)
+        outer = [after["fo"], after["io"]]
+        inner = [after["fy"], after["iy"]]
+        assert all(t in (1, 2, 4, 8, 16, 32, 64) for t in outer), after
+        if tt == "2D":
+            assert inner == [0, 0], after
+        else:
+            assert all(t in (1, 2, 4, 8, 16, 32, 64) for t in inner), after
+        for _ in range(2):
+            y1 = host(op(dx))
+            assert np.array_equal(y1, y0), (after, rel(y1, y0))
+
+
+def test_autotune_errors(mr):
+    from mr_aha_amd import _lib
+    from oracle import conventions as cv
+    g = cv.make_geometry("2D", (40, 32), 2, 2)
+    cfg = mr.make_config("2D", (40, 32), 2, n_coils=2)
+    K = torch.zeros((g.T,) + g.grid_shape(), dtype=torch.complex64, device="cuda")
+    S = torch.ones((2,) + g.img_shape(), dtype=torch.complex64, device="cuda")
+    x = torch.ones((2,) + g.img_shape(), dtype=torch.complex64, device="cuda")
+    out = torch.empty_like(x)
+    with mr.create_op(cfg, S, None, None, _phi2_toep=K) as op:
+        rc = _lib.lib.mr_op_autotune(op._handle, x.data_ptr(), out.data_ptr(), 0)
+        assert rc == -1  # MR_ERR_INVALID_ARG
+        assert b"reps 0" in _lib.lib.mr_last_error()
+        rc = _lib.lib.mr_op_autotune(op._handle, x.data_ptr(), x.data_ptr(), 1)
+        assert rc == -1  # MR_ERR_INVALID_ARG
+        assert b"alias" in _lib.lib.mr_last_error()
+        t = (ctypes.c_int * 5)()
+        assert _lib.lib.mr_op_tiles(op._handle, t, 5) == 0
