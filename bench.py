@@ -267,6 +267,9 @@ def main():
                          "16-bit kernel; serve: request mix, p50/p99 latency")
     ap.add_argument("--requests", type=int, default=24, help="serve: number of requests")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in tile rules instead of timing the candidates "
+                         "(mr_op_autotune) before the warm-up")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16"],
                     help="kernel storage read by the applies (16-bit: per-4x4-block scales)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -319,6 +322,12 @@ def main():
         op.set_kernel_format(args.precision)
     x = w["x"]
     out = torch.empty_like(x)
+    tiles = None
+    if not args.no_autotune:
+        t_tune = time.perf_counter()
+        tiles = op.autotune(x, reps=2)
+        print(f"[bench] autotuned in {time.perf_counter() - t_tune:.1f} s: {tiles}",
+              file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         op.apply(x, out)
     torch.cuda.synchronize()
@@ -378,7 +387,8 @@ def main():
                    "coil_group": group,
                    "pipeline": (f"fused x pass, {xbd} coil(s) per workgroup" if xbd
                                 else "separate FFT and block passes"),
-                   "kernel_build_s": round(info["toep_ms"] / 1e3, 2)},
+                   "kernel_build_s": round(info["toep_ms"] / 1e3, 2),
+                   "tiles": tiles if tiles is not None else "built-in rules"},
         "roofline": {"bound": "hbm", "kernel": dom_kernel, "stage": dom,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
