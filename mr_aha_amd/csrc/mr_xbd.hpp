@@ -166,6 +166,139 @@ __device__ __forceinline__ void mr_bd_walk_pair(float2 *a0, float2 *a1, int ms, 
     }
 }
 
+// mr_bd_walk_pair with the kernel entries fetched two per 16-byte load. The
+// wavefront holds 32 pairs of adjacent locations (lane l < 32: the even one,
+// lane l + 32: the odd one). For a column's rows j, j+1 the low half loads row
+// j's entries of both locations of its pair and the high half row j+1's
+// (boff_pair: the pair's even-location offset, plus one column for the high
+// half); v_permlane32_swap then leaves every lane with both entries of its own
+// location. An odd column length ends with one 8-byte load at boff_own. The
+// loads, which bound the fused pass, drop from T to about T/2 per location.
+// Needs the pair's entries adjacent and 16-byte aligned (an even kernel index
+// for the even location).
+//
+// Load unit u covers rows j, j+1 (or the last row j) of walk index i; units
+// are consumed in walk order through a ring of D, and the walk is unrolled
+// by template recursion over u so that every index is a constant.
+template <int L> struct mr_x2_units {
+    static constexpr int count()
+    {
+        int n = 0;
+        for (int i = 0; i < L; ++i) n += (L - i + 1) / 2;
+        return n;
+    }
+    static constexpr int walk(int u)
+    {
+        int i = 0;
+        while (u >= (L - i + 1) / 2) u -= (L - i + 1) / 2, ++i;
+        return i;
+    }
+    static constexpr int row(int u)
+    {
+        int i = 0;
+        while (u >= (L - i + 1) / 2) u -= (L - i + 1) / 2, ++i;
+        return 2 * u;
+    }
+};
+
+typedef unsigned int mr_u32x4 __attribute__((ext_vector_type(4)));
+
+template <int L, bool UPPER, int U>
+__device__ __forceinline__ mr_u32x4 mr_x2_load(const float2 *ker, long long ks, unsigned bp,
+                                               unsigned bo)
+{
+    constexpr int i = mr_x2_units<L>::walk(U), j = mr_x2_units<L>::row(U);
+    constexpr int c = UPPER ? L - 1 - i : i, len = L - i;
+    constexpr int t0 = UPPER ? c * (c + 1) / 2 : c * L - c * (c - 1) / 2;
+    const __amdgpu_buffer_rsrc_t rs = mr_rsrc(ker + ks * t0, 0xFFFFFFFFu);
+    if constexpr (j + 1 < len) {
+        return __builtin_amdgcn_raw_buffer_load_b128(rs, (int)bp, (int)(ks * 8 * j), 0);
+    } else {
+        const mr_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)bo, (int)(ks * 8 * j), 0);
+        return mr_u32x4{v.x, v.y, 0u, 0u};
+    }
+}
+
+template <int L, bool UPPER, int D, int U>
+__device__ __forceinline__ void mr_x2_step(float2 (&y0)[L], float2 (&y1)[L], mr_u32x4 (&kq)[D],
+                                           float2 *a0, float2 *a1, int ms, const float2 *ker,
+                                           long long ks, unsigned bp, unsigned bo, float2 &vc0,
+                                           float2 &vc1)
+{
+    constexpr int NU = mr_x2_units<L>::count();
+    if constexpr (U < NU) {
+        constexpr int i = mr_x2_units<L>::walk(U), j = mr_x2_units<L>::row(U);
+        constexpr int c = UPPER ? L - 1 - i : i, len = L - i;
+        if constexpr (j == 0) {
+            vc0 = a0[c * ms];
+            vc1 = a1[c * ms];
+        }
+        const mr_u32x4 q = kq[U % D];
+        if constexpr (U + D < NU) kq[U % D] = mr_x2_load<L, UPPER, U + D>(ker, ks, bp, bo);
+        float2 e[2];
+        if constexpr (j + 1 < len) {
+            // lanes 0-31 hold row j of the pair, 32-63 row j+1: after the
+            // swap every lane has rows j and j+1 of its own location
+            const auto sx = __builtin_amdgcn_permlane32_swap(q.x, q.z, false, false);
+            const auto sy = __builtin_amdgcn_permlane32_swap(q.y, q.w, false, false);
+            e[0] = make_float2(__uint_as_float(sx[0]), __uint_as_float(sy[0]));
+            e[1] = make_float2(__uint_as_float(sx[1]), __uint_as_float(sy[1]));
+        } else {
+            e[0] = make_float2(__uint_as_float(q.x), __uint_as_float(q.y));
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (j + h < len) {
+                const int r = (UPPER ? 0 : c) + j + h;
+                if (r == c) {
+                    y0[c] = cmac(y0[c], e[h], vc0);
+                    y1[c] = cmac(y1[c], e[h], vc1);
+                } else {
+                    y0[c] = cmac(y0[c], e[h], a0[r * ms]);
+                    y0[r] = cmacc(y0[r], e[h], vc0);
+                    y1[c] = cmac(y1[c], e[h], a1[r * ms]);
+                    y1[r] = cmacc(y1[r], e[h], vc1);
+                }
+                // consumed before the next unit is requested: at most D units
+                // are held in registers
+                asm volatile("" : "+v"(y0[c].x), "+v"(y0[c].y), "+v"(y0[r].x), "+v"(y0[r].y),
+                             "+v"(y1[c].x), "+v"(y1[c].y), "+v"(y1[r].x), "+v"(y1[r].y)::"memory");
+            }
+        }
+        if constexpr (j + 2 >= len) {
+            a0[c * ms] = y0[c];
+            a1[c * ms] = y1[c];
+        }
+        mr_x2_step<L, UPPER, D, U + 1>(y0, y1, kq, a0, a1, ms, ker, ks, bp, bo, vc0, vc1);
+    }
+}
+
+template <int L, bool UPPER, int D, int U = 0>
+__device__ __forceinline__ void mr_x2_prefetch(mr_u32x4 (&kq)[D], const float2 *ker, long long ks,
+                                               unsigned bp, unsigned bo)
+{
+    if constexpr (U < D && U < mr_x2_units<L>::count()) {
+        kq[U] = mr_x2_load<L, UPPER, U>(ker, ks, bp, bo);
+        mr_x2_prefetch<L, UPPER, D, U + 1>(kq, ker, ks, bp, bo);
+    }
+}
+
+template <int L, bool UPPER, int D>
+__device__ __forceinline__ void mr_bd_walk_pair_x2(float2 *a0, float2 *a1, int ms, const float2 *ker,
+                                                   long long ks, unsigned boff_pair,
+                                                   unsigned boff_own)
+{
+    float2 y0[L], y1[L], vc0, vc1;
+    mr_u32x4 kq[D];
+    mr_x2_prefetch<L, UPPER, D>(kq, ker, ks, boff_pair, boff_own);
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        y0[l] = make_float2(0.f, 0.f);
+        y1[l] = make_float2(0.f, 0.f);
+    }
+    mr_x2_step<L, UPPER, D, 0>(y0, y1, kq, a0, a1, ms, ker, ks, boff_pair, boff_own, vc0, vc1);
+}
+
 // ---------------------------------------------------------------------------
 // 16-bit kernel format (BF16 or FP16 entries with FP32 scales per 4x4 block)
 // ---------------------------------------------------------------------------

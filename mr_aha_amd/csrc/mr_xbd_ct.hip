@@ -123,8 +123,19 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
         // 16-bit kernel, half the bytes per entry, measured no gain from it)
         const int npair = gn >> 1;
         const int nitem = ((gn + 1) >> 1) * N;
+        // 16-byte entry loads over lane pairs (mr_bd_walk_pair_x2): unmasked,
+        // so that a pair of adjacent locations has adjacent kernel entries
+        // (N and the kx centering shift even: the pair never straddles the
+        // kx wrap, and its even entry index is even) and a 16-byte aligned
+        // kernel. Lane l of a 64-location block then takes location 2l
+        // (l < 32) or 2(l - 32) + 1.
+        const bool x2 = !MASKED && N % 64 == 0 && d.c_m[0] % 2 == 0 &&
+                        ((unsigned long long)ker & 15) == 0;
         auto item = [&](int wi) {
-            const int q = (int)mr_cdiv<N>::q((unsigned)wi), e = wi - q * N;
+            const int q = (int)mr_cdiv<N>::q((unsigned)wi);
+            int e = wi - q * N;
+            const int hi = (e & 63) >> 5;
+            if (x2 && q < npair) e = (e & ~63) | (((e & 31) << 1) + hi);
             const long long b = kidx(e);
             float2 *a0 = A + 2 * q * L * N + e;
             if (q < npair) {
@@ -135,6 +146,12 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
                         a0[l * N] = make_float2(0.f, 0.f);
                         a1[l * N] = make_float2(0.f, 0.f);
                     }
+                } else if (!MASKED && x2) {
+                    const unsigned bp = (unsigned)((b - hi) * 8 + (hi ? kstride * 8 : 0));
+                    if (d.tri_upper)
+                        mr_bd_walk_pair_x2<L, true, 4>(a0, a1, N, ker, kstride, bp, (unsigned)(b * 8));
+                    else
+                        mr_bd_walk_pair_x2<L, false, 4>(a0, a1, N, ker, kstride, bp, (unsigned)(b * 8));
                 } else if (d.tri_upper) {
                     mr_bd_walk_pair<L, true, (L > 8 ? 8 : 16)>(a0, a1, N, ker, kstride, (unsigned)(b * 8));
                 } else {
