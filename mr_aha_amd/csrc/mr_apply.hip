@@ -931,6 +931,7 @@ static int mr_blockdiag(mr_aha_plan p, float2 *K, int nc, const float2 *ker)
     mr_bd_fn fn = masked ? mr_bd_pick<true>(d.L, TP) : mr_bd_pick<false>(d.L, TP);
     if (fn) {
         const long long blocks = mr_ceil_div(d.pad_size, TP);
+        MR_GRID_CHECK(blocks, TP * slots, "mr_k_bd");
         hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(TP * slots), 0, p->stream, K, ker,
                            p->d_inv, kstride, d, nc);
         MR_LAUNCH_CHECK("mr_k_bd");
@@ -940,6 +941,7 @@ static int mr_blockdiag(mr_aha_plan p, float2 *K, int nc, const float2 *ker)
     if (lds > 64 * 1024)
         MR_FAIL(MR_ERR_UNSUPPORTED, "L = %d needs %zu bytes of LDS per 64 locations", d.L, lds);
     const long long blocks = mr_ceil_div(d.pad_size, 64);
+    MR_GRID_CHECK(blocks, 64, "mr_k_bd_any");
     if (masked)
         hipLaunchKernelGGL(mr_k_bd_any<true>, dim3((unsigned)blocks), dim3(64), lds, p->stream, K,
                            ker, p->d_inv, kstride, d, nc);
@@ -979,10 +981,10 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
         mr_zmap zm{V, lgT};
         mr_ld_sx_z ld{x, S, zm, plane, N, No, Mo, cno, L, coil0};
         mr_st_z st{A, zm, plane, Mo};
-        // pencils per workgroup: the autotuned width, else 2 (measured fastest
-        // for 56-point z at 320x320); MR_AHA_HIP_FO_TX overrides both, and
-        // likewise for the other passes below
-        const int txo = (int)mr_env_ll("MR_AHA_HIP_FO_TX", p->tile[0] > 0 ? p->tile[0] : 2);
+        // pencils per workgroup: the autotuned width, else 2^lgT up to 64;
+        // MR_AHA_HIP_FO_TX overrides both, and likewise for the other passes
+        // below
+        const int txo = (int)mr_env_ll("MR_AHA_HIP_FO_TX", p->tile[0] > 0 ? p->tile[0] : 1 << lgT);
         MR_TRY((mr_fft_pass_ct<false>(fo, ld, st, plane * V, 0, txo, s, "Fo")));
         MR_TRY(mr_prof_mark(p, two ? MR_ST_F2 : MR_ST_F3));
     }
@@ -1003,6 +1005,7 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
         if (blocks > 0x7fffffffLL)
             MR_FAIL(MR_ERR_UNSUPPORTED, "fused x pass: %lld workgroups exceed the launch limit",
                     blocks);
+        MR_GRID_CHECK(blocks, p->xbd_ct ? p->xbd_nt : MR_XBD_THREADS, "fused x pass");
         if (p->xbd_ct) {
             hipLaunchKernelGGL(p->xbd_ct, dim3((unsigned)blocks), dim3((unsigned)p->xbd_nt), p->xbd_lds,
                                s, B, ker, p->d_inv, kstride, d, p->fx.tws, R, G, nc, ncg, p->kq,

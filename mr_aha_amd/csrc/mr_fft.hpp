@@ -481,7 +481,8 @@ __device__ __forceinline__ void mr_fft_store_tile(const Store &st, typename Stor
 // workgroup (the last one may hold fewer).
 template <typename R, bool INV, class Load, class Store>
 __global__ void __launch_bounds__(256)
-mr_fft_axis_kernel(mr_fft_len<R> f, Load ld, Store st, long long npencil, int lgTX)
+mr_fft_axis_kernel(mr_fft_len<R> f, Load ld, Store st, long long npencil, int lgTX,
+                   long long pbase)
 {
     using V = typename cvec<R>::T;
     extern __shared__ __align__(16) unsigned char mr_smem[];
@@ -494,7 +495,7 @@ mr_fft_axis_kernel(mr_fft_len<R> f, Load ld, Store st, long long npencil, int lg
     V *A0 = tw + n;
     V *B0 = A0 + (size_t)TX * ns;
     V *acc = B0 + (size_t)TX * ns;
-    const long long p0 = mr_xcd_block(blockIdx.x, gridDim.x) * TX;
+    const long long p0 = pbase + mr_xcd_block(blockIdx.x, gridDim.x) * TX;
     const int tp = (int)((npencil - p0 < TX) ? (npencil - p0) : TX);
     for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = f.tws[i];
     const int nsub = ld.nsub();
@@ -570,7 +571,8 @@ __device__ __forceinline__ void mr_stage_reg(V *A, const V *tw, int n, int ns, i
 
 template <typename R, bool INV, class Load, class Store>
 __global__ void __launch_bounds__(MR_FFT_FAST_THREADS)
-mr_fft_axis_fast(mr_fft_len<R> f, Load ld, Store st, long long npencil, int lgTX)
+mr_fft_axis_fast(mr_fft_len<R> f, Load ld, Store st, long long npencil, int lgTX,
+                 long long pbase)
 {
     using V = typename cvec<R>::T;
     extern __shared__ __align__(16) unsigned char mr_smem[];
@@ -582,7 +584,7 @@ mr_fft_axis_fast(mr_fft_len<R> f, Load ld, Store st, long long npencil, int lgTX
     V *tw = reinterpret_cast<V *>(mr_smem);
     V *A = tw + n;
     V *acc = A + (size_t)TX * ns;
-    const long long p0 = mr_xcd_block(blockIdx.x, gridDim.x) * TX;
+    const long long p0 = pbase + mr_xcd_block(blockIdx.x, gridDim.x) * TX;
     const int tp = (int)((npencil - p0 < TX) ? (npencil - p0) : TX);
     for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = f.tws[i];
     const int nsub = ld.nsub();

@@ -180,7 +180,8 @@ template <int N> constexpr int mr_ct_axis_tpmax()
 
 template <int N, bool INV, class Load, class Store>
 __global__ void __launch_bounds__(MR_FFT_FAST_THREADS)
-mr_fft_axis_ct(const float2 *tws, Load ld, Store st, long long npencil, int lgTX)
+mr_fft_axis_ct(const float2 *tws, Load ld, Store st, long long npencil, int lgTX,
+               long long pbase)
 {
     constexpr int NT = MR_FFT_FAST_THREADS;
     constexpr int NSTR = (N % 2 == 0) ? N + 1 : N;
@@ -192,7 +193,7 @@ mr_fft_axis_ct(const float2 *tws, Load ld, Store st, long long npencil, int lgTX
     float2 *twl = reinterpret_cast<float2 *>(mr_smem);
     float2 *A = twl + N;
     float2 *acc = A + (size_t)TX * NSTR;
-    const long long p0 = mr_xcd_block(blockIdx.x, gridDim.x) * TX;
+    const long long p0 = pbase + mr_xcd_block(blockIdx.x, gridDim.x) * TX;
     const int tp = (int)((npencil - p0 < TX) ? (npencil - p0) : TX);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int i = threadIdx.x; i < N; i += NT) twl[i] = tws[i];
@@ -253,11 +254,16 @@ int mr_fft_pass_ct_n(const mr_fft_len<float> &f, const Load &ld, const Store &st
     int lg = 0;
     while ((1 << lg) < TX) ++lg;
     const long long blocks = (npencil + TX - 1) / TX;
-    if (blocks > 0x7fffffffLL)
-        MR_FAIL(MR_ERR_UNSUPPORTED, "%s: %lld pencils exceed the launch limit", name, npencil);
-    hipLaunchKernelGGL((mr_fft_axis_ct<N, INV, Load, Store>), dim3((unsigned)blocks),
-                       dim3(MR_FFT_FAST_THREADS), need(TX), stream, f.tws, ld, st, npencil, lg);
-    MR_LAUNCH_CHECK(name);
+    // a dispatch counts its work-items in 32 bits: larger grids are launched
+    // in chunks of whole tiles (pbase: first pencil of the chunk)
+    const long long maxb = mr_max_blocks(MR_FFT_FAST_THREADS);
+    for (long long b0 = 0; b0 < blocks; b0 += maxb) {
+        const long long nb = blocks - b0 < maxb ? blocks - b0 : maxb;
+        hipLaunchKernelGGL((mr_fft_axis_ct<N, INV, Load, Store>), dim3((unsigned)nb),
+                           dim3(MR_FFT_FAST_THREADS), need(TX), stream, f.tws, ld, st, npencil, lg,
+                           b0 * TX);
+        MR_LAUNCH_CHECK(name);
+    }
     return MR_SUCCESS;
 }
 

@@ -1,6 +1,8 @@
 // HIP-side helpers shared by the device translation units.
 #pragma once
 
+#include <cstdlib>
+
 #include "mr_internal.hpp"
 #include "mr_fft.hpp"
 
@@ -20,6 +22,32 @@
         if (mr_e_ != hipSuccess)                                               \
             MR_FAIL(MR_ERR_CUDA, "kernel %s failed to launch at %s:%d: %s",    \
                     name, __FILE__, __LINE__, hipGetErrorString(mr_e_));       \
+    } while (0)
+
+// Most workgroups of `threads` work-items one dispatch may hold: the dispatch
+// packet counts work-items in 32 bits, and a larger grid would be silently
+// truncated. Kept a multiple of 1024 workgroups. MR_AHA_HIP_MAX_BLOCKS lowers
+// it (tests run the chunked launches at small sizes with it).
+inline long long mr_hw_max_blocks(int threads)
+{
+    return (0xFFFFFFFFLL / threads) & ~1023LL;
+}
+inline long long mr_max_blocks(int threads)
+{
+    const long long hw = mr_hw_max_blocks(threads);
+    const char *e = getenv("MR_AHA_HIP_MAX_BLOCKS");
+    const long long v = e ? atoll(e) : 0;
+    return (v > 0 && v < hw) ? v : hw;
+}
+
+// Fail (instead of launching a truncated grid) when blocks x threads
+// work-items do not fit one dispatch; for launches that are not chunked.
+#define MR_GRID_CHECK(blocks, threads, name)                                   \
+    do {                                                                       \
+        if ((long long)(blocks) > mr_hw_max_blocks(threads))                   \
+            MR_FAIL(MR_ERR_UNSUPPORTED,                                        \
+                    "%s: %lld workgroups of %d exceed one dispatch (2^32 "     \
+                    "work-items)", name, (long long)(blocks), (int)(threads)); \
     } while (0)
 
 // Make cfg's device current (gpu_device_id < 0 keeps the caller's); *prev
@@ -63,15 +91,20 @@ int mr_fft_pass(const mr_fft_len<R> &f, const Load &ld, const Store &st, long lo
     int lg = 0;
     while ((1 << lg) < TX) ++lg;
     const long long blocks = (npencil + TX - 1) / TX;
-    if (blocks > 0x7fffffffLL)
-        MR_FAIL(MR_ERR_UNSUPPORTED, "%s: %lld pencils exceed the launch limit", name, npencil);
-    if (smooth)
-        hipLaunchKernelGGL((mr_fft_axis_fast<R, INV, Load, Store>), dim3((unsigned)blocks),
-                           dim3(MR_FFT_FAST_THREADS), lds, stream, f, ld, st, npencil, lg);
-    else
-        hipLaunchKernelGGL((mr_fft_axis_kernel<R, INV, Load, Store>), dim3((unsigned)blocks),
-                           dim3(256), lds, stream, f, ld, st, npencil, lg);
-    MR_LAUNCH_CHECK(name);
+    // a dispatch counts its work-items in 32 bits: larger grids are launched
+    // in chunks of whole tiles (pbase: first pencil of the chunk)
+    const int threads = smooth ? MR_FFT_FAST_THREADS : 256;
+    const long long maxb = mr_max_blocks(threads);
+    for (long long b0 = 0; b0 < blocks; b0 += maxb) {
+        const long long nb = blocks - b0 < maxb ? blocks - b0 : maxb;
+        if (smooth)
+            hipLaunchKernelGGL((mr_fft_axis_fast<R, INV, Load, Store>), dim3((unsigned)nb),
+                               dim3(threads), lds, stream, f, ld, st, npencil, lg, b0 * TX);
+        else
+            hipLaunchKernelGGL((mr_fft_axis_kernel<R, INV, Load, Store>), dim3((unsigned)nb),
+                               dim3(threads), lds, stream, f, ld, st, npencil, lg, b0 * TX);
+        MR_LAUNCH_CHECK(name);
+    }
     return MR_SUCCESS;
 }
 

@@ -258,6 +258,7 @@ int mr_calc_toep_impl(const mr_config *cfg, const mr_complex *d_phi2, const floa
         const int qn = (int)((Q - q0 < Qc) ? Q - q0 : Qc);
         MR_TRY(mr_grid_spread(g, smp, b.c + q0 * Ntrajs, Ntrajs, qn, b.fine, s));
         MR_TRY(mr_grid_fft(g, b.fine, nullptr, qn, false, s));
+        MR_GRID_CHECK(mr_ceil_div(tg.Ptr * qn, 256), 256, "mr_k_deconv");
         hipLaunchKernelGGL(mr_k_deconv, dim3((unsigned)mr_ceil_div(tg.Ptr * qn, 256)), dim3(256),
                            0, s, b.fine, g, tg.Ptr, qn, b.dc[0], b.dc[1], b.dc[2], b.nat);
         MR_LAUNCH_CHECK("mr_k_deconv");
@@ -309,6 +310,7 @@ static int mr_calc_toep_from_psf_impl(const mr_config *cfg, const mr_complex *d_
         MR_FAIL(MR_ERR_ALLOC, "failed to allocate %lld scratch elements for mr_calc_toep_from_psf",
                 tg.Ptr * Qc);
     }
+    MR_GRID_CHECK(mr_ceil_div(tg.Ptr * Qc, 256), 256, "mr_k_psf_to_nat");
     hipLaunchKernelGGL(mr_k_psf_to_nat, dim3((unsigned)mr_ceil_div(tg.Ptr * Qc, 256)), dim3(256),
                        0, s, reinterpret_cast<const float2 *>(d_psf), tg.M[0], tg.M[1], tg.M[2],
                        tg.shift[0], tg.shift[1], tg.shift[2], tg.Ptr, (int)Qc, b.nat);

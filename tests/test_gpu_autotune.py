@@ -100,3 +100,56 @@ def test_autotune_errors(mr):
         assert b"alias" in _lib.lib.mr_last_error()
         t = (ctypes.c_int * 5)()
         assert _lib.lib.mr_op_tiles(op._handle, t, 5) == 0
+
+
+# Dispatch size: a grid of more than 2^32 work-items is launched in chunks
+# (a single dispatch would be truncated without an error).
+
+@pytest.mark.parametrize("tt,n,L,C", [("stack", (48, 40, 10), 4, 3), ("3D", (24, 20, 16), 4, 2),
+                                      ("2D", (60, 44), 8, 3), ("2D", (320, 12), 16, 2)])
+def test_chunked_launches_match(mr, monkeypatch, tt, n, L, C):
+    """Every axis pass split into dispatches of 7 workgroups gives the same
+    bits as single dispatches."""
+    from oracle import conventions as cv
+    print(f"[chunks] {tt} {n} L={L} C={C}", flush=True)
+    rng = np.random.default_rng(3 * L + C)
+    g = cv.make_geometry(tt, n, L, C)
+    kshape = (g.T,) + g.grid_shape()
+    K = (rng.standard_normal(kshape) + 1j * rng.standard_normal(kshape)).astype(np.complex64)
+    x = (rng.standard_normal((L,) + g.img_shape())
+         + 1j * rng.standard_normal((L,) + g.img_shape())).astype(np.complex64)
+    S = (rng.standard_normal((C,) + g.img_shape())
+         + 1j * rng.standard_normal((C,) + g.img_shape())).astype(np.complex64)
+    cfg = mr.make_config(g.traj_type, g.n[: g.n_spatial], L, phi2_dim=g.M[: g.n_spatial],
+                         n_coils=C)
+    dx, dS, dK = dev(x), dev(S), dev(K)
+    with mr.create_op(cfg, dS, None, None, _phi2_toep=dK) as op:
+        y0 = host(op(dx))
+        monkeypatch.setenv("MR_AHA_HIP_MAX_BLOCKS", "7")
+        y1 = host(op(dx))
+        monkeypatch.delenv("MR_AHA_HIP_MAX_BLOCKS")
+    assert np.array_equal(y0, y1), rel(y1, y0)
+
+
+def test_outer_pass_beyond_32bit_grid(mr, monkeypatch):
+    """160x160x8 stack, L=16, 21 coils: the outer-axis pass at one pencil per
+    workgroup has 8.6M workgroups of 512 work-items (more than 2^32). Its
+    result must equal the same pass at 64 pencils per workgroup."""
+    print("[chunks] stack 160x160x8 L16 C21, one pencil per workgroup", flush=True)
+    n, nz, L, C = 160, 8, 16, 21
+    cfg = mr.make_config("stack", (n, n, nz), L, n_coils=C, mem_fraction=0.9)
+    T = L * (L + 1) // 2
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    K = torch.randn((T, nz, 2 * n, 2 * n), dtype=torch.complex64, device="cuda", generator=gen)
+    S = torch.randn((C, nz, n, n), dtype=torch.complex64, device="cuda", generator=gen)
+    x = torch.randn((L, nz, n, n), dtype=torch.complex64, device="cuda", generator=gen)
+    with mr.create_op(cfg, S, None, None, _phi2_toep=K) as op:
+        assert op.info["group"][0] == C
+        assert n * n * L * C > (0xFFFFFFFF // 512)  # pencils of the outer pass at TX = 1
+        monkeypatch.setenv("MR_AHA_HIP_FO_TX", "64")
+        y64 = op(x)
+        monkeypatch.setenv("MR_AHA_HIP_FO_TX", "1")
+        y1 = op(x)
+        monkeypatch.delenv("MR_AHA_HIP_FO_TX")
+        torch.cuda.synchronize()
+        assert torch.equal(y1, y64), float((y1 - y64).abs().max())
