@@ -58,6 +58,40 @@ __device__ __forceinline__ void mr_xbd_load_src(float2 *Ap, __amdgpu_buffer_rsrc
     }
 }
 
+// The same with two points per lane and 16-byte loads and LDS writes (chunks
+// of 128 points): for Nx and cnx even, points s, s+1 (s even) land in the
+// adjacent slots e, e+1 (e even). Lanes past the Nx points write nothing.
+typedef unsigned int mr_u32x4_ld __attribute__((ext_vector_type(4)));
+template <int N, int NC>
+__device__ __forceinline__ void mr_xbd_load_chunks2(float2 *Ap, __amdgpu_buffer_rsrc_t rs, int lane,
+                                                    int cnx, int Nx)
+{
+    mr_u32x4_ld v[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+        v[c] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)((c * 128 + 2 * lane) * 8), 0, 0);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int s = c * 128 + 2 * lane;
+        int e = s - cnx;
+        if (e < 0) e += N;
+        if (s < Nx) *reinterpret_cast<mr_u32x4_ld *>(Ap + e) = v[c];
+    }
+}
+
+template <int N, int NC = (N + 127) / 128>
+__device__ __forceinline__ void mr_xbd_load_src2(float2 *Ap, __amdgpu_buffer_rsrc_t rs, int lane,
+                                                 int cnx, int Nx, int nc)
+{
+    if constexpr (NC >= 1) {
+        if (nc == NC) {
+            mr_xbd_load_chunks2<N, NC>(Ap, rs, lane, cnx, Nx);
+            return;
+        }
+        mr_xbd_load_src2<N, NC - 1>(Ap, rs, lane, cnx, Nx, nc);
+    }
+}
+
 // NT threads, up to GMAX coils per workgroup (the default: two workgroups of
 // 512 threads per CU; the wide form: 1024 threads and two coils in 160 KB,
 // one workgroup per CU). Either way a CU holds 16 wavefronts.
@@ -91,10 +125,16 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
     // instructions, which are what bounds this kernel); the N - Nx padding
     // slots e = (s - cnx) mod N, s >= Nx, are written with zeros.
     const int nck = (Nx + 63) >> 6;
+    // two points per lane when the pencils and their slots pair up (Nx a
+    // multiple of 4, so Nx and cnx = Nx/2 are even; 16-byte aligned rows)
+    const bool px = (Nx & 3) == 0 && ((unsigned long long)Y & 15) == 0;
     for (int pl = wave; pl < tp; pl += NW) {
         const __amdgpu_buffer_rsrc_t rs = mr_rsrc(Yr + pstride * pl, rowbytes);
         float2 *Ap = A + pl * N;
-        mr_xbd_load_src<N>(Ap, rs, lane, cnx, nck);
+        if (px)
+            mr_xbd_load_src2<N>(Ap, rs, lane, cnx, Nx, (Nx + 127) >> 7);
+        else
+            mr_xbd_load_src<N>(Ap, rs, lane, cnx, nck);
         for (int i = lane; i < N - Nx; i += 64) {
             int e = Nx + i - cnx;
             if (e >= N) e -= N;
@@ -199,10 +239,19 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
     for (int pl = wave; pl < tp; pl += NW) {
         const __amdgpu_buffer_rsrc_t rs = mr_rsrc(Yr + pstride * pl, rowbytes);
         const float2 *Ap = A + pl * N;
-        for (int s = lane; s < Nx; s += 64) {
-            int e = s - cnx;
-            if (e < 0) e += N;
-            mr_bstore(rs, (unsigned)s * 8u, Ap[e]);
+        if (px) {
+            for (int s = 2 * lane; s < Nx; s += 128) {
+                int e = s - cnx;
+                if (e < 0) e += N;
+                __builtin_amdgcn_raw_buffer_store_b128(
+                    *reinterpret_cast<const mr_u32x4_ld *>(Ap + e), rs, s * 8, 0, 0);
+            }
+        } else {
+            for (int s = lane; s < Nx; s += 64) {
+                int e = s - cnx;
+                if (e < 0) e += N;
+                mr_bstore(rs, (unsigned)s * 8u, Ap[e]);
+            }
         }
     }
 }
