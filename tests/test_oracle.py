@@ -22,7 +22,7 @@ def test_known_answers():
     assert cv.num_tri(16) == 136 and cv.num_tri(0) == 0 and cv.num_tri(1) == 1
     g = cv.make_geometry("stack", (320, 320, 56), 16, 12)
     assert g.M == (640, 640, 56)
-    assert g.P * g.T == 3_119_513_600                       # mr_phi2_toep_size, config 2
+    assert g.P * g.T == 3_119_513_600  # mr_phi2_toep_size of the 320x320x56, L=16 stack
     g1 = cv.make_geometry("2D", (128, 128), 4, 4)
     assert g1.P * g1.T == 655_360
 
