@@ -417,3 +417,39 @@ def test_nufft_method(mr, golden, name):
     with mr.create_op(cfg, dS, dphi2, dtraj, method="nufft",
                       nufft_opts={"upsampfac": 2.0, "gpu_method": 1}) as op:
         assert rel(host(op(dx)), gd["AhAx_nudft"]) < 1e-5
+
+
+# fused x pass variants: pencil I/O one point per lane (x length 2 mod 4) or
+# two (0 mod 4); kernel entries two per 16-byte load (16-byte aligned kernel)
+# or one per 8-byte load (kernel pointer offset by 8 bytes)
+FUSED_PATHS = [((126, 10), (256, 20), 16, 2, False), ((250, 8), (640, 16), 16, 2, False),
+               ((160, 8), (320, 16), 16, 3, True), ((320, 12), (640, 24), 16, 2, True),
+               ((96, 12), (384, 24), 8, 3, True)]
+
+
+@pytest.mark.parametrize("n,M,L,C,offset", FUSED_PATHS)
+def test_fused_pair_paths(mr, n, M, L, C, offset):
+    from oracle import conventions as cv
+    from oracle import toeplitz
+    print(f"[fused paths] n={n} M={M} L={L} C={C} offset={offset}", flush=True)
+    rng = np.random.default_rng(sum(n) + L + C)
+    g = cv.make_geometry("2D", n, L, C, phi2_dim=M)
+    kshape = (g.T,) + g.grid_shape()
+    K = (rng.standard_normal(kshape) + 1j * rng.standard_normal(kshape)).astype(np.complex64)
+    x = (rng.standard_normal((L,) + g.img_shape())
+         + 1j * rng.standard_normal((L,) + g.img_shape())).astype(np.complex64)
+    S = (rng.standard_normal((C,) + g.img_shape())
+         + 1j * rng.standard_normal((C,) + g.img_shape())).astype(np.complex64)
+    if offset:
+        buf = torch.empty(K.size + 1, dtype=torch.complex64, device="cuda")
+        dK = buf[1:].view(kshape)
+        dK.copy_(dev(K))
+        assert dK.data_ptr() % 16 == 8
+    else:
+        dK = dev(K)
+    for tri in ("lower", "upper"):
+        cfg = cfg_for(mr, g, triangulation=tri)
+        assert _xbd_group(mr, cfg) > 0
+        ref = toeplitz.apply(g, x, S, K.astype(np.complex128), upper=(tri == "upper"))
+        y = host(mr.aha_toeplitz(cfg, dev(x), dev(S), dK, None))
+        assert rel(y, ref) < 1e-5, (tri, rel(y, ref))
