@@ -219,7 +219,7 @@ __device__ __forceinline__ mr_u32x4 mr_x2_load(const float2 *ker, long long ks, 
     }
 }
 
-template <int L, bool UPPER, int D, int U>
+template <int L, bool UPPER, int D, int U, int NC>
 __device__ __forceinline__ void mr_x2_step(float2 (&y0)[L], float2 (&y1)[L], mr_u32x4 (&kq)[D],
                                            float2 *a0, float2 *a1, int ms, const float2 *ker,
                                            long long ks, unsigned bp, unsigned bo, float2 &vc0,
@@ -231,7 +231,7 @@ __device__ __forceinline__ void mr_x2_step(float2 (&y0)[L], float2 (&y1)[L], mr_
         constexpr int c = UPPER ? L - 1 - i : i, len = L - i;
         if constexpr (j == 0) {
             vc0 = a0[c * ms];
-            vc1 = a1[c * ms];
+            if constexpr (NC == 2) vc1 = a1[c * ms];
         }
         const mr_u32x4 q = kq[U % D];
         if constexpr (U + D < NU) kq[U % D] = mr_x2_load<L, UPPER, U + D>(ker, ks, bp, bo);
@@ -252,24 +252,29 @@ __device__ __forceinline__ void mr_x2_step(float2 (&y0)[L], float2 (&y1)[L], mr_
                 const int r = (UPPER ? 0 : c) + j + h;
                 if (r == c) {
                     y0[c] = cmac(y0[c], e[h], vc0);
-                    y1[c] = cmac(y1[c], e[h], vc1);
+                    if constexpr (NC == 2) y1[c] = cmac(y1[c], e[h], vc1);
                 } else {
                     y0[c] = cmac(y0[c], e[h], a0[r * ms]);
                     y0[r] = cmacc(y0[r], e[h], vc0);
-                    y1[c] = cmac(y1[c], e[h], a1[r * ms]);
-                    y1[r] = cmacc(y1[r], e[h], vc1);
+                    if constexpr (NC == 2) {
+                        y1[c] = cmac(y1[c], e[h], a1[r * ms]);
+                        y1[r] = cmacc(y1[r], e[h], vc1);
+                    }
                 }
                 // consumed before the next unit is requested: at most D units
                 // are held in registers
-                asm volatile("" : "+v"(y0[c].x), "+v"(y0[c].y), "+v"(y0[r].x), "+v"(y0[r].y),
-                             "+v"(y1[c].x), "+v"(y1[c].y), "+v"(y1[r].x), "+v"(y1[r].y)::"memory");
+                if constexpr (NC == 2)
+                    asm volatile("" : "+v"(y0[c].x), "+v"(y0[c].y), "+v"(y0[r].x), "+v"(y0[r].y),
+                                 "+v"(y1[c].x), "+v"(y1[c].y), "+v"(y1[r].x), "+v"(y1[r].y)::"memory");
+                else
+                    asm volatile("" : "+v"(y0[c].x), "+v"(y0[c].y), "+v"(y0[r].x), "+v"(y0[r].y)::"memory");
             }
         }
         if constexpr (j + 2 >= len) {
             a0[c * ms] = y0[c];
-            a1[c * ms] = y1[c];
+            if constexpr (NC == 2) a1[c * ms] = y1[c];
         }
-        mr_x2_step<L, UPPER, D, U + 1>(y0, y1, kq, a0, a1, ms, ker, ks, bp, bo, vc0, vc1);
+        mr_x2_step<L, UPPER, D, U + 1, NC>(y0, y1, kq, a0, a1, ms, ker, ks, bp, bo, vc0, vc1);
     }
 }
 
@@ -283,12 +288,13 @@ __device__ __forceinline__ void mr_x2_prefetch(mr_u32x4 (&kq)[D], const float2 *
     }
 }
 
-template <int L, bool UPPER, int D>
+// NC = 2: coils a0 and a1 share every load; NC = 1: a0 only (a1 unused).
+template <int L, bool UPPER, int D, int NC = 2>
 __device__ __forceinline__ void mr_bd_walk_pair_x2(float2 *a0, float2 *a1, int ms, const float2 *ker,
                                                    long long ks, unsigned boff_pair,
                                                    unsigned boff_own)
 {
-    float2 y0[L], y1[L], vc0, vc1;
+    float2 y0[L], y1[L], vc0, vc1 = make_float2(0.f, 0.f);
     mr_u32x4 kq[D];
     mr_x2_prefetch<L, UPPER, D>(kq, ker, ks, boff_pair, boff_own);
 #pragma unroll
@@ -296,7 +302,7 @@ __device__ __forceinline__ void mr_bd_walk_pair_x2(float2 *a0, float2 *a1, int m
         y0[l] = make_float2(0.f, 0.f);
         y1[l] = make_float2(0.f, 0.f);
     }
-    mr_x2_step<L, UPPER, D, 0>(y0, y1, kq, a0, a1, ms, ker, ks, boff_pair, boff_own, vc0, vc1);
+    mr_x2_step<L, UPPER, D, 0, NC>(y0, y1, kq, a0, a1, ms, ker, ks, boff_pair, boff_own, vc0, vc1);
 }
 
 // ---------------------------------------------------------------------------

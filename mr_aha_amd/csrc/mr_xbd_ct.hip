@@ -214,13 +214,26 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
             for (int wi = threadIdx.x; wi < nitem; wi += NT) item(wi);
         }
     } else {
+    // FP32, one walk per coil: entries two per 16-byte load over lane pairs
+    // under the conditions of the pair walk
+    const bool x1 = KF == MR_KF_FP32 && !MASKED && N % 64 == 0 && d.c_m[0] % 2 == 0 &&
+                    ((unsigned long long)ker & 15) == 0;
     for (int wi = threadIdx.x; wi < gn * N; wi += NT) {
-        const int g = (int)mr_cdiv<N>::q((unsigned)wi), e = wi - g * N;
+        const int g = (int)mr_cdiv<N>::q((unsigned)wi);
+        int e = wi - g * N;
+        const int hi = (e & 63) >> 5;
+        if (x1) e = (e & ~63) | (((e & 31) << 1) + hi);
         const long long b = kidx(e);
         float2 *a = A + g * L * N + e;
         if (b < 0) {
 #pragma unroll
             for (int l = 0; l < L; ++l) a[l * N] = make_float2(0.f, 0.f);
+        } else if (KF == MR_KF_FP32 && !MASKED && x1) {
+            const unsigned bp = (unsigned)((b - hi) * 8 + (hi ? kstride * 8 : 0));
+            if (d.tri_upper)
+                mr_bd_walk_pair_x2<L, true, 8, 1>(a, a, N, ker, kstride, bp, (unsigned)(b * 8));
+            else
+                mr_bd_walk_pair_x2<L, false, 8, 1>(a, a, N, ker, kstride, bp, (unsigned)(b * 8));
         } else if constexpr (KF != MR_KF_FP32) {
             if (d.tri_upper)
                 mr_bd_walk_q<L, true, KF>(a, N, kq, ksc, kstride, (unsigned)(b * 4), (unsigned)(b * 4));

@@ -424,14 +424,19 @@ def test_nufft_method(mr, golden, name):
 # or one per 8-byte load (kernel pointer offset by 8 bytes)
 FUSED_PATHS = [((126, 10), (256, 20), 16, 2, False), ((250, 8), (640, 16), 16, 2, False),
                ((160, 8), (320, 16), 16, 3, True), ((320, 12), (640, 24), 16, 2, True),
-               ((96, 12), (384, 24), 8, 3, True)]
+               ((96, 12), (384, 24), 8, 3, True), ((320, 12), (640, 24), 16, 1, False)]
 
 
+@pytest.mark.parametrize("wide", [True, False])
 @pytest.mark.parametrize("n,M,L,C,offset", FUSED_PATHS)
-def test_fused_pair_paths(mr, n, M, L, C, offset):
+def test_fused_pair_paths(mr, monkeypatch, n, M, L, C, offset, wide):
+    """wide=False keeps one coil per workgroup (the 16-byte entry loads then
+    serve one coil's walk)."""
     from oracle import conventions as cv
     from oracle import toeplitz
-    print(f"[fused paths] n={n} M={M} L={L} C={C} offset={offset}", flush=True)
+    print(f"[fused paths] n={n} M={M} L={L} C={C} offset={offset} wide={wide}", flush=True)
+    if not wide:
+        monkeypatch.setenv("MR_AHA_HIP_XBD_WIDE", "0")
     rng = np.random.default_rng(sum(n) + L + C)
     g = cv.make_geometry("2D", n, L, C, phi2_dim=M)
     kshape = (g.T,) + g.grid_shape()
