@@ -8,6 +8,8 @@
 
 #include "mr_hip.hpp"
 
+#include <type_traits>
+
 // Stage radices of n by the rule of mr_fft_factor (8s, then 10, 6, 4, 2, 3, 5, 7)
 // and the product Ns of the radices before each stage.
 struct mr_ct_fac {
@@ -96,7 +98,13 @@ struct mr_tw_lds {
 // test on the wavefront's first butterfly); inside a round, lanes past the
 // last butterfly repeat it, reading the same inputs and writing the same
 // values to the same places, so no per-lane mask lives across the barrier.
-template <int N, int NSTR, int R, int NS, bool INV, int NT, int TPMAX, class TW>
+// SWI / SWO: the stage reads / writes element e of pencil p at e ^ mr_ct_swz(p)
+// (the wide-tile layout, whose unpadded pencils would otherwise put the 16
+// pencils of one row on the same LDS banks); inner stages use plain order.
+__device__ __forceinline__ unsigned mr_ct_swz(unsigned p) { return (p & 7u) << 2; }
+
+template <int N, int NSTR, int R, int NS, bool INV, int NT, int TPMAX, class TW, bool SWI = false,
+          bool SWO = false>
 __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int wave)
 {
     constexpr int m = N / R;
@@ -112,8 +120,14 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
             const unsigned f = min((unsigned)(threadIdx.x + k * NT), (unsigned)(nb - 1));
             const unsigned p = mr_cdiv<m>::q(f), j = f - p * m;
             const float2 *a = A + p * NSTR + j;
+            if constexpr (SWI) {
+                const unsigned c = mr_ct_swz(p);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[k][r] = a[r * m];
+                for (int r = 0; r < R; ++r) v[k][r] = A[p * NSTR + ((j + r * m) ^ c)];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[k][r] = a[r * m];
+            }
             if constexpr (NS > 1) {
                 const unsigned kk = j - mr_cdiv<NS>::q(j) * NS;
                 if constexpr (TW::kPowers) {
@@ -150,21 +164,29 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
             }
             bfly<R, INV>(v[k]);
             float2 *b = A + p * NSTR + jn * (NS * R) + kk;
+            if constexpr (SWO) {
+                const unsigned c = mr_ct_swz(p), o = jn * (NS * R) + kk;
 #pragma unroll
-            for (int q = 0; q < R; ++q) b[q * NS] = v[k][q];
+                for (int q = 0; q < R; ++q) A[p * NSTR + ((o + q * NS) ^ c)] = v[k][q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < R; ++q) b[q * NS] = v[k][q];
+            }
         }
     }
     __syncthreads();
 }
 
-template <int N, int NSTR, int S, bool INV, int NT, int TPMAX, class TW>
+template <int N, int NSTR, int S, bool INV, int NT, int TPMAX, class TW, bool SW = false>
 __device__ __forceinline__ void mr_ct_fft(float2 *A, const TW &tw, int tp, int wave)
 {
     constexpr mr_ct_fac F = mr_ct_factor(N);
     static_assert(F.rest == 1, "length must be 2^a 3^b 5^c 7^d");
+    static_assert(!SW || (N % 32 == 0 && F.nst >= 2), "swizzled order needs 32 | N, 2 stages");
     if constexpr (S < F.nst) {
-        mr_ct_stage<N, NSTR, F.r[S], F.ns[S], INV, NT, TPMAX>(A, tw, tp, wave);
-        mr_ct_fft<N, NSTR, S + 1, INV, NT, TPMAX>(A, tw, tp, wave);
+        mr_ct_stage<N, NSTR, F.r[S], F.ns[S], INV, NT, TPMAX, TW, SW && S == 0,
+                    SW && S == F.nst - 1>(A, tw, tp, wave);
+        mr_ct_fft<N, NSTR, S + 1, INV, NT, TPMAX, TW, SW>(A, tw, tp, wave);
     }
 }
 
@@ -178,27 +200,69 @@ template <int N> constexpr int mr_ct_axis_tpmax()
     return t;
 }
 
-template <int N, bool INV, class Load, class Store>
+// W16 (wide tile): 16 pencils per workgroup, so that a pencil group spans
+// whole 128-byte lines when pencils are the contiguous axis. It holds only the
+// pencils in LDS (stride N, 16·N·8 bytes: two workgroups per CU at N = 640)
+// and reads twiddles from the global table (powers formed by products).
+template <int N, bool W16> struct mr_ct_axis_form {
+    static constexpr int NSTR = W16 ? N : ((N % 2 == 0) ? N + 1 : N);
+    static constexpr int TPMAX = W16 ? 16 : mr_ct_axis_tpmax<N>();
+    static constexpr int NTW = W16 ? 0 : N;  // twiddles held in LDS
+};
+
+template <int N, bool INV, class Load, class Store, bool W16 = false>
 __global__ void __launch_bounds__(MR_FFT_FAST_THREADS)
 mr_fft_axis_ct(const float2 *tws, Load ld, Store st, long long npencil, int lgTX,
                long long pbase)
 {
     constexpr int NT = MR_FFT_FAST_THREADS;
-    constexpr int NSTR = (N % 2 == 0) ? N + 1 : N;
-    constexpr int TPMAX = mr_ct_axis_tpmax<N>();
+    using FM = mr_ct_axis_form<N, W16>;
+    constexpr int NSTR = FM::NSTR;
+    constexpr int TPMAX = FM::TPMAX;
     extern __shared__ __align__(16) unsigned char mr_smem[];
     __shared__ typename Load::PB lpb[MR_FFT_MAXTX];
     __shared__ typename Store::PB spb[MR_FFT_MAXTX];
     const int TX = 1 << lgTX;
     float2 *twl = reinterpret_cast<float2 *>(mr_smem);
-    float2 *A = twl + N;
+    float2 *A = twl + FM::NTW;
     float2 *acc = A + (size_t)TX * NSTR;
     const long long p0 = pbase + mr_xcd_block(blockIdx.x, gridDim.x) * TX;
     const int tp = (int)((npencil - p0 < TX) ? (npencil - p0) : TX);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int i = threadIdx.x; i < N; i += NT) twl[i] = tws[i];
-    const mr_tw_lds tw{twl};
+    using TWT = typename std::conditional<W16, mr_tw_buf, mr_tw_lds>::type;
+    TWT tw;
+    if constexpr (W16) {
+        tw = mr_tw_buf{mr_rsrc(tws, (unsigned)N * 8u)};
+    } else {
+        for (int i = threadIdx.x; i < N; i += NT) twl[i] = tws[i];
+        tw = mr_tw_lds{twl};
+    }
     const int nsub = ld.nsub();
+    if constexpr (W16) {
+        // thread t always serves pencil t mod 16 (NT is a multiple of 16):
+        // its pencil descriptors stay in registers, and 16 lanes read or
+        // write one 128-byte line of each row
+        static_assert(!Load::kElemFast && !Store::kElemFast && NT % 16 == 0, "wide tile form");
+        const int p = threadIdx.x & 15;
+        const unsigned c = mr_ct_swz(p);
+        const bool pv = p < tp;
+        const long long pp = p0 + (pv ? p : 0);
+        for (int sub = 0; sub < nsub; ++sub) {
+            const typename Load::PB lb = ld.pencil(pp, sub);
+            const typename Store::PB sb = st.pencil(pp, sub);
+            if (pv)
+                for (int e = threadIdx.x >> 4; e < N; e += NT / 16)
+                    A[p * NSTR + (e ^ c)] = ld.get(lb, e);
+            __syncthreads();
+            mr_ct_fft<N, NSTR, 0, INV, NT, TPMAX, TWT, true>(A, tw, tp, wave);
+            if (pv)
+                for (int e = threadIdx.x >> 4; e < N; e += NT / 16)
+                    st.put(sb, p, e, sub, A[p * NSTR + (e ^ c)], acc);
+            __syncthreads();
+        }
+        st.flush(p0, tp, acc);
+        return;
+    }
     for (int sub = 0; sub < nsub; ++sub) {
         if (threadIdx.x < tp) {
             lpb[threadIdx.x] = ld.pencil(p0 + threadIdx.x, sub);
@@ -246,6 +310,32 @@ int mr_fft_pass_ct_n(const mr_fft_len<float> &f, const Load &ld, const Store &st
     int TX = want_tx > MR_FFT_MAXTX ? MR_FFT_MAXTX : want_tx;
     const long long txmax = mr_env_ll("MR_AHA_HIP_FFT_TXMAX", MR_FFT_MAXTX);
     if (TX > txmax) TX = (int)(txmax < 1 ? 1 : txmax);
+    // a dispatch counts its work-items in 32 bits: larger grids are launched
+    // in chunks of whole tiles (pbase: first pencil of the chunk)
+    const long long maxb = mr_max_blocks(MR_FFT_FAST_THREADS);
+    // wide tile: 16 pencils per workgroup where a tile of 16 is asked for, no
+    // per-pencil accumulator is needed and 16 pencils take at most 80 KB
+    constexpr size_t need16 = (size_t)16 * N * sizeof(float2);
+    if constexpr (N >= 256 && need16 <= 80 * 1024 && !Load::kElemFast && !Store::kElemFast) {
+        // dynamic LDS above 64 KB must be allowed per kernel, once
+        static const bool w16_ok =
+            hipFuncSetAttribute(reinterpret_cast<const void *>(
+                                    &mr_fft_axis_ct<N, INV, Load, Store, true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)need16) == hipSuccess;
+        if (w16_ok && TX >= 16 && acc_per_pencil == 0 && npencil >= 16 &&
+            mr_env_ll("MR_AHA_HIP_FFT_W16", 1) != 0) {
+        const long long blocks = (npencil + 15) / 16;
+        for (long long b0 = 0; b0 < blocks; b0 += maxb) {
+            const long long nb = blocks - b0 < maxb ? blocks - b0 : maxb;
+            hipLaunchKernelGGL((mr_fft_axis_ct<N, INV, Load, Store, true>), dim3((unsigned)nb),
+                               dim3(MR_FFT_FAST_THREADS), need16, stream, f.tws, ld, st, npencil,
+                               4, b0 * 16);
+            MR_LAUNCH_CHECK(name);
+        }
+        return MR_SUCCESS;
+        }
+    }
     auto need = [&](int tx) {
         return ((size_t)N + (size_t)tx * NSTR + (size_t)tx * acc_per_pencil) * sizeof(float2);
     };
@@ -254,9 +344,6 @@ int mr_fft_pass_ct_n(const mr_fft_len<float> &f, const Load &ld, const Store &st
     int lg = 0;
     while ((1 << lg) < TX) ++lg;
     const long long blocks = (npencil + TX - 1) / TX;
-    // a dispatch counts its work-items in 32 bits: larger grids are launched
-    // in chunks of whole tiles (pbase: first pencil of the chunk)
-    const long long maxb = mr_max_blocks(MR_FFT_FAST_THREADS);
     for (long long b0 = 0; b0 < blocks; b0 += maxb) {
         const long long nb = blocks - b0 < maxb ? blocks - b0 : maxb;
         hipLaunchKernelGGL((mr_fft_axis_ct<N, INV, Load, Store>), dim3((unsigned)nb),

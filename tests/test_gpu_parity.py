@@ -256,7 +256,7 @@ def test_fused_specialised_lengths(mr, monkeypatch, tt, n, L, C, masked):
 
 
 CT_AXIS = [("stack", (20, 128, 56), 4, 2), ("3D", (20, 16, 160), 4, 2), ("2D", (24, 160), 8, 3),
-           ("2D", (40, 192), 4, 2), ("stack", (18, 256, 8), 8, 2)]
+           ("2D", (40, 192), 4, 2), ("stack", (18, 256, 8), 8, 2), ("stack", (21, 128, 5), 3, 1)]
 
 
 @pytest.mark.parametrize("tt,n,L,C", CT_AXIS)
@@ -264,7 +264,10 @@ def test_fft_compile_time_lengths(mr, monkeypatch, tt, n, L, C):
     """y and z grid lengths 256, 56, 320, 384 and 512 run the axis passes
     compiled for their length (outer-axis pass with the sensitivity product,
     y pass with padding, inverse y pass with crop, inverse outer pass with the
-    coil sum); against the FP64 apply and the run-time-length passes."""
+    coil sum); against the FP64 apply and the run-time-length passes. The y
+    passes run both in 16-pencil tiles (swizzled LDS order, twiddles from the
+    global table; (21, 128, 5) leaves a partial last tile) and in the padded
+    tiles of at most 8 pencils."""
     from oracle import conventions as cv
     from oracle import toeplitz
     print(f"[ct-axis] {tt} {n} L={L} C={C}", flush=True)
@@ -277,12 +280,13 @@ def test_fft_compile_time_lengths(mr, monkeypatch, tt, n, L, C):
     cfg = cfg_for(mr, g)
     ref = toeplitz.apply(g, x, S, K.astype(np.complex128))
     outs = []
-    for ct in ("1", "0"):
+    for ct, w16 in (("1", "1"), ("1", "0"), ("0", "1")):
         monkeypatch.setenv("MR_AHA_HIP_FFT_CT", ct)
+        monkeypatch.setenv("MR_AHA_HIP_FFT_W16", w16)
         y = host(mr.aha_toeplitz(cfg, dev(x), dev(S), dev(K)))
-        assert rel(y, ref) < 1e-5, (ct, rel(y, ref))
+        assert rel(y, ref) < 1e-5, (ct, w16, rel(y, ref))
         outs.append(y)
-    assert rel(outs[0], outs[1]) < 1e-6
+    assert rel(outs[0], outs[2]) < 1e-6 and rel(outs[1], outs[2]) < 1e-6
 
 
 @pytest.mark.parametrize("L", [1, 2, 5, 7, 12, 13, 16, 17, 20])
