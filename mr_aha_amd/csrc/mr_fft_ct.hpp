@@ -200,6 +200,11 @@ template <int N> constexpr int mr_ct_axis_tpmax()
     return t;
 }
 
+// Threads of the 16-pencil tile: 1024 from length 512 up (32 wavefronts per
+// CU at two workgroups), 512 below, where 1024 would leave half the lanes of
+// the butterfly rounds idle.
+template <int N> constexpr int mr_w16_threads() { return N >= 512 ? 1024 : 512; }
+
 // W16 (wide tile): 16 pencils per workgroup, so that a pencil group spans
 // whole 128-byte lines when pencils are the contiguous axis. It holds only the
 // pencils in LDS (stride N, 16·N·8 bytes: two workgroups per CU at N = 640)
@@ -211,11 +216,11 @@ template <int N, bool W16> struct mr_ct_axis_form {
 };
 
 template <int N, bool INV, class Load, class Store, bool W16 = false>
-__global__ void __launch_bounds__(MR_FFT_FAST_THREADS)
+__global__ void __launch_bounds__(W16 ? mr_w16_threads<N>() : MR_FFT_FAST_THREADS)
 mr_fft_axis_ct(const float2 *tws, Load ld, Store st, long long npencil, int lgTX,
                long long pbase)
 {
-    constexpr int NT = MR_FFT_FAST_THREADS;
+    constexpr int NT = W16 ? mr_w16_threads<N>() : MR_FFT_FAST_THREADS;
     using FM = mr_ct_axis_form<N, W16>;
     constexpr int NSTR = FM::NSTR;
     constexpr int TPMAX = FM::TPMAX;
@@ -326,10 +331,11 @@ int mr_fft_pass_ct_n(const mr_fft_len<float> &f, const Load &ld, const Store &st
         if (w16_ok && TX >= 16 && acc_per_pencil == 0 && npencil >= 16 &&
             mr_env_ll("MR_AHA_HIP_FFT_W16", 1) != 0) {
         const long long blocks = (npencil + 15) / 16;
-        for (long long b0 = 0; b0 < blocks; b0 += maxb) {
-            const long long nb = blocks - b0 < maxb ? blocks - b0 : maxb;
+        const long long maxb16 = mr_max_blocks(mr_w16_threads<N>());
+        for (long long b0 = 0; b0 < blocks; b0 += maxb16) {
+            const long long nb = blocks - b0 < maxb16 ? blocks - b0 : maxb16;
             hipLaunchKernelGGL((mr_fft_axis_ct<N, INV, Load, Store, true>), dim3((unsigned)nb),
-                               dim3(MR_FFT_FAST_THREADS), need16, stream, f.tws, ld, st, npencil,
+                               dim3(mr_w16_threads<N>()), need16, stream, f.tws, ld, st, npencil,
                                4, b0 * 16);
             MR_LAUNCH_CHECK(name);
         }
