@@ -256,7 +256,8 @@ def test_fused_specialised_lengths(mr, monkeypatch, tt, n, L, C, masked):
 
 
 CT_AXIS = [("stack", (20, 128, 56), 4, 2), ("3D", (20, 16, 160), 4, 2), ("2D", (24, 160), 8, 3),
-           ("2D", (40, 192), 4, 2), ("stack", (18, 256, 8), 8, 2), ("stack", (21, 128, 5), 3, 1)]
+           ("2D", (40, 192), 4, 2), ("stack", (18, 256, 8), 8, 2), ("stack", (21, 128, 5), 3, 1),
+           ("stack", (21, 320, 4), 3, 2)]
 
 
 @pytest.mark.parametrize("tt,n,L,C", CT_AXIS)
@@ -266,8 +267,9 @@ def test_fft_compile_time_lengths(mr, monkeypatch, tt, n, L, C):
     y pass with padding, inverse y pass with crop, inverse outer pass with the
     coil sum); against the FP64 apply and the run-time-length passes. The y
     passes run both in 16-pencil tiles (swizzled LDS order, twiddles from the
-    global table; (21, 128, 5) leaves a partial last tile) and in the padded
-    tiles of at most 8 pencils."""
+    global table; 1024 threads at y grid 512 and 640; (21, 128, 5) and
+    (21, 320, 4) leave a partial last tile) and in the padded tiles of at most
+    8 pencils."""
     from oracle import conventions as cv
     from oracle import toeplitz
     print(f"[ct-axis] {tt} {n} L={L} C={C}", flush=True)
