@@ -268,6 +268,29 @@ mr_fft_axis_ct(const float2 *tws, Load ld, Store st, long long npencil, int lgTX
         st.flush(p0, tp, acc);
         return;
     }
+    if constexpr (!Load::kElemFast && !Store::kElemFast) {
+        // pencils are the contiguous axis: thread t always serves pencil
+        // t mod TX (NT is a multiple of TX), so its descriptors stay in
+        // registers; the elements each thread moves are those of the loops
+        // below
+        const int p = threadIdx.x & (TX - 1);
+        const bool pv = p < tp;
+        const long long pp = p0 + (pv ? p : 0);
+        const int e0 = threadIdx.x >> lgTX, de = NT >> lgTX;
+        for (int sub = 0; sub < nsub; ++sub) {
+            const typename Load::PB lb = ld.pencil(pp, sub);
+            const typename Store::PB sb = st.pencil(pp, sub);
+            if (pv)
+                for (int e = e0; e < N; e += de) A[p * NSTR + e] = ld.get(lb, e);
+            __syncthreads();
+            mr_ct_fft<N, NSTR, 0, INV, NT, TPMAX>(A, tw, tp, wave);
+            if (pv)
+                for (int e = e0; e < N; e += de) st.put(sb, p, e, sub, A[p * NSTR + e], acc);
+            __syncthreads();
+        }
+        st.flush(p0, tp, acc);
+        return;
+    }
     for (int sub = 0; sub < nsub; ++sub) {
         if (threadIdx.x < tp) {
             lpb[threadIdx.x] = ld.pencil(p0 + threadIdx.x, sub);
