@@ -338,6 +338,9 @@ int mr_fft_pass_ct_n(const mr_fft_len<float> &f, const Load &ld, const Store &st
     int TX = want_tx > MR_FFT_MAXTX ? MR_FFT_MAXTX : want_tx;
     const long long txmax = mr_env_ll("MR_AHA_HIP_FFT_TXMAX", MR_FFT_MAXTX);
     if (TX > txmax) TX = (int)(txmax < 1 ? 1 : txmax);
+    // 2^lgTX pencils per workgroup: round a width that is not a power of two
+    // down before LDS and the grid are sized for it
+    TX = mr_pow2_floor(TX);
     // a dispatch counts its work-items in 32 bits: larger grids are launched
     // in chunks of whole tiles (pbase: first pencil of the chunk)
     const long long maxb = mr_max_blocks(MR_FFT_FAST_THREADS);

@@ -50,6 +50,14 @@ inline long long mr_max_blocks(int threads)
                     "work-items)", name, (long long)(blocks), (int)(threads)); \
     } while (0)
 
+// Largest power of two <= v (1 for v < 1).
+inline int mr_pow2_floor(int v)
+{
+    int p = 1;
+    while (p <= v / 2) p *= 2;
+    return p;
+}
+
 // Make cfg's device current (gpu_device_id < 0 keeps the caller's); *prev
 // receives the device to restore or -1.
 int mr_push_device(const mr_config *cfg, int *prev);
@@ -68,6 +76,9 @@ int mr_fft_pass(const mr_fft_len<R> &f, const Load &ld, const Store &st, long lo
     using V = typename cvec<R>::T;
     if (npencil <= 0) return MR_SUCCESS;
     if (want_tx > MR_FFT_MAXTX) want_tx = MR_FFT_MAXTX;
+    // the kernels take 2^lgTX pencils per workgroup: a width that is not a
+    // power of two is rounded down here, before LDS and the grid are sized
+    want_tx = mr_pow2_floor(want_tx);
     static const bool fast_on = mr_env_ll("MR_AHA_HIP_FFT_FAST", 1) != 0;
     bool smooth = sizeof(R) == 4 && fast_on;
     for (int i = 0; i < f.nst && smooth; ++i) smooth = mr_fft_unrolled(f.radix[i]);

@@ -86,6 +86,81 @@ def calc_kernel(g: Geometry, phi2: np.ndarray, traj: np.ndarray, upper: bool = F
     return K
 
 
+def _dirichlet_cols(x: np.ndarray, kappas: np.ndarray, M: int) -> np.ndarray:
+    """D_M(2 pi kappa/M - x_m) for samples x (S,) and signed coordinates
+    kappas (K,), shape (S, K): the closed form
+    e^{-i th (M - 2 ceil(M/2) + 1)/2} sin(M th/2) / sin(th/2), and the direct
+    sum over the M modes where |sin(th/2)| < 1e-6 (the closed form's
+    removable singularity). Same function as ``dirichlet_matrix``."""
+    th = 2.0 * np.pi * kappas[None, :].astype(np.float64) / M - x[:, None].astype(np.float64)
+    s = np.sin(th / 2)
+    shift = (M - 2 * ((M + 1) // 2) + 1) / 2.0
+    near = np.abs(s) < 1e-6
+    out = np.exp(-1j * th * shift) * np.sin(M * th / 2) / np.where(near, 1.0, s)
+    if near.any():
+        d = np.arange(M) - M // 2
+        out[near] = np.exp(1j * np.outer(th[near], d)).sum(axis=1)
+    return out
+
+
+def kernel_entries(g: Geometry, phi2: np.ndarray, traj: np.ndarray, sel: np.ndarray,
+                   upper: bool = False, chunk: int = 1 << 18) -> np.ndarray:
+    """Exact entries of ``phi2_toep`` at a few locations, without the full grid.
+
+    ``sel`` is an int array (E, 1 + n_spatial) of (t, [kz,] ky, kx) centered
+    indices, the row-major index order of ``calc_kernel``'s result. Returns
+    complex128 (E,). Same sum as ``calc_kernel`` (per transformed axis a
+    Dirichlet factor of the signed coordinate kappa_a = k_a - floor(M_a/2),
+    strengths constant along a readout, scale 1/prod M_tr), evaluated per
+    entry over all samples in chunks, so the full-size workloads can be
+    checked at sampled entries. Cheap when the entries share few distinct
+    coordinates per axis (each distinct value costs one pass over the samples).
+
+    Pinned against ``calc_kernel`` at every entry of small 2D, stack and 3D
+    grids with odd and even M (tests/test_oracle.py).
+    """
+    sel = np.asarray(sel, dtype=np.int64)
+    Mx, My, Mz = g.M
+    c = strengths(phi2, g, upper)            # (T, Ntr) or (T, Nkz, Ntr)
+    ndim, Nkx, Ntr = traj.shape
+    three = g.traj_type == "3D"
+    if three:
+        t, kz, ky, kx = sel.T
+        axes = [(kx, Mx, traj[0]), (ky, My, traj[1]), (kz, Mz, traj[2])]
+        scale = 1.0 / (Mx * My * Mz)
+    else:
+        if g.traj_type == "stack":
+            t, kz, ky, kx = sel.T
+        else:
+            t, ky, kx = sel.T
+            kz = np.zeros_like(t)
+        axes = [(kx, Mx, traj[0]), (ky, My, traj[1])]
+        scale = 1.0 / (Mx * My)
+    # distinct coordinate tuples: their per-readout Dirichlet products G[s]
+    coords = np.stack([a[0] - a[1] // 2 for a in axes], axis=1)     # signed kappa per entry
+    uniq, inv = np.unique(coords, axis=0, return_inverse=True)
+    inv = inv.ravel()
+    G = np.zeros((len(uniq), Ntr), dtype=np.complex128)
+    xs = [np.asarray(a[2], dtype=np.float64).ravel() for a in axes]  # sample m = r*Ntr + s
+    nsamp = Nkx * Ntr
+    for m0 in range(0, nsamp, chunk):
+        m1 = min(nsamp, m0 + chunk)
+        prod = np.ones((m1 - m0, len(uniq)), dtype=np.complex128)
+        for ai, (_, M, _) in enumerate(axes):
+            vals, vinv = np.unique(uniq[:, ai], return_inverse=True)
+            D = _dirichlet_cols(xs[ai][m0:m1], vals, M)
+            prod *= D[:, vinv.ravel()]
+        s_idx = np.arange(m0, m1) % Ntr
+        for u in range(len(uniq)):
+            G[u] += np.bincount(s_idx, weights=prod[:, u].real, minlength=Ntr) + \
+                1j * np.bincount(s_idx, weights=prod[:, u].imag, minlength=Ntr)
+    if g.traj_type == "stack":
+        cs = c[t, kz]                        # (E, Ntr)
+    else:
+        cs = c[t]
+    return scale * np.einsum("es,es->e", cs, G[inv])
+
+
 def _img_to_grid_index(g: Geometry):
     """Natural padded indices of every image index, per row-major axis."""
     return [pad_index(np.arange(g.n[a]), g.n[a], g.M[a]) for a in reversed(range(g.n_spatial))]

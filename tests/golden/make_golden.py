@@ -50,10 +50,26 @@ def case_inputs(name: str):
     if name == "radial_128_L4_C4":
         g, traj, phi2, x, SEs, _ = gen.radial_workload()
         return g, traj, phi2, x, SEs
+    if name in RECIPE_CASES:
+        g, traj, phi2, x, SEs, _ = RECIPE_CASES[name]()
+        return g, traj, phi2, x, SEs
     return gen.reference_case(name)
 
 
-CASES = [c[0] for c in gen.REFERENCE_CASES] + ["checkop_2d", "checkop_stack", "radial_128_L4_C4"]
+#: The benchmark recipes at reduced size: stack-of-stars with L=16, C=12, a
+#: 200-frame orthonormal basis assigned round-robin over golden-angle spokes
+#: (the 320x320x56 workload's recipe at 40x40x8, 48 spokes x 80 samples), and
+#: the Fibonacci-sphere kooshball with L=8, C=8, a 100-frame basis (the 160^3
+#: workload's recipe at 24^3, 400 spokes x 48 samples).
+RECIPE_CASES = {
+    "stack_40x40x8_L16_C12_T200": lambda: gen.stack_workload(
+        n=40, nz=8, L=16, C=12, n_spokes=48, n_kx=80, n_frames=200, seed=20261205),
+    "koosh_24_L8_C8_T100": lambda: gen.kooshball_workload(
+        n=24, L=8, C=8, n_spokes=400, n_kx=48, n_frames=100, seed=20261206),
+}
+
+CASES = [c[0] for c in gen.REFERENCE_CASES] + ["checkop_2d", "checkop_stack",
+                                               "radial_128_L4_C4"] + list(RECIPE_CASES)
 
 
 def make(name: str) -> Path:

@@ -122,6 +122,23 @@ def test_toeplitz_equals_nudft(case):
         assert rel(toeplitz.apply(g, x, S, K, upper=up), ref) < 1e-12
 
 
+@pytest.mark.parametrize("case", SMALL)
+def test_kernel_entries_match_full_kernel(case):
+    """Sampled exact entries == the full exact kernel at every entry, with a
+    sample planted at the origin (a removable singularity of the closed form)."""
+    g, traj, phi2, x, S = _small(*case)
+    traj = traj.copy()
+    traj[:, 0, 0] = 0.0
+    for up in (False, True):
+        K = toeplitz.calc_kernel(g, phi2, traj, upper=up)
+        idx = np.array(np.unravel_index(np.arange(K.size), K.shape)).T
+        E = toeplitz.kernel_entries(g, phi2, traj, idx, upper=up, chunk=13)
+        assert rel(E, K.ravel()) < 1e-12
+    # a conjugated Gram block moves the entries (the check is not vacuous)
+    E = toeplitz.kernel_entries(g, np.conj(phi2), traj, idx)
+    assert rel(E, toeplitz.calc_kernel(g, phi2, traj).ravel()) > 1e-3
+
+
 def test_planted_mistakes_fail():
     """A conjugated kernel, a transposed Gram block or a flipped NUFFT sign
     must all be caught by the Toeplitz-vs-NUDFT check."""
@@ -288,7 +305,8 @@ def test_generators_reference_draws():
 
 
 @pytest.mark.parametrize("name", ["2d_33x27_L1_c2_odd", "2d_50x40_L3_c3", "checkop_2d",
-                                  "stack_31x36x9_L4_c2_odd"])
+                                  "stack_31x36x9_L4_c2_odd", "stack_40x40x8_L16_C12_T200",
+                                  "koosh_24_L8_C8_T100"])
 def test_golden_regression(golden, name):
     from tests.golden.make_golden import case_inputs, input_checksum
     gd = golden(name)
