@@ -54,7 +54,9 @@ MR_EXPORT int mr_op_autotune(mr_op op, const mr_complex *d_x, mr_complex *d_AhAx
 /* Tiling in use: tiles[0..3] = pencils per workgroup requested for the
  * outer-axis, y, inverse y and inverse outer-axis passes (0: the built-in
  * rule), tiles[4] = 1 when the fused x pass runs two-coil 1024-thread
- * workgroups. Writes min(n, 5) entries. No counterpart in the reference. */
+ * workgroups, tiles[5] = 1 when its block step runs on the matrix cores (for
+ * a 16-byte aligned kernel). Writes min(n, 6) entries. No counterpart in the
+ * reference. */
 MR_EXPORT int mr_aha_plan_tiles(mr_aha_plan plan, int *tiles, int n);
 MR_EXPORT int mr_op_tiles(mr_op op, int *tiles, int n);
 
@@ -94,6 +96,7 @@ MR_EXPORT int mr_op_set_kernel_format(mr_op op, int format);
 #define MR_CG_GRAPH 2
 MR_EXPORT int mr_op_cg(mr_op op, mr_op op_refresh, const mr_complex *d_b, mr_complex *d_x,
                        double lambda, int n_iter, int refresh, int flags, double *res_hist);
+
 
 #ifdef __cplusplus
 }

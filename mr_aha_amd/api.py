@@ -520,11 +520,12 @@ class AhAOp:
     def tiles(self) -> dict:
         """Tiling in use (library entry mr_op_tiles): pencils per workgroup
         requested for the outer-axis, y, inverse y and inverse outer-axis FFT
-        passes (0: the built-in rule) and whether the fused x pass runs
-        two-coil workgroups."""
-        t = (ctypes.c_int * 5)()
-        check(lib.mr_op_tiles(self._open(), t, 5), "mr_op_tiles")
-        return dict(fo=t[0], fy=t[1], iy=t[2], io=t[3], wide=bool(t[4]))
+        passes (0: the built-in rule), whether the fused x pass runs
+        two-coil workgroups and whether its block step runs on the matrix
+        cores."""
+        t = (ctypes.c_int * 6)()
+        check(lib.mr_op_tiles(self._open(), t, 6), "mr_op_tiles")
+        return dict(fo=t[0], fy=t[1], iy=t[2], io=t[3], wide=bool(t[4]), mfma=bool(t[5]))
 
     def cg(self, b, x=None, *, lam: float = 0.0, iters: int = 40, refresh: int = 10,
            op_refresh: Optional["AhAOp"] = None, graph: bool = False):

@@ -619,6 +619,8 @@ struct mr_aha_plan_s {
     int xbd_nt;           // its workgroup size
     int tile[4];          // pencils per workgroup of Fo, Fy, Iy, Io (0: default rule)
     int no_wide;          // 1: no two-coil 1024-thread fused workgroups
+    mr_xbd_ct_fn xbd_mf;  // xbd_ct with the block step on the matrix cores, or null
+    int no_mf;            // 1: keep the FMA block walks (autotuner's choice)
     bool tuning;          // mr_aha_plan_autotune is timing stages
     int kf;               // kernel format used by execute (MR_KF_*)
     const unsigned *kq;   // 16-bit entries and block scales (kf != FP32)
@@ -726,6 +728,7 @@ static int mr_xbd_setup(mr_aha_plan p)
     p->xbd_g = 0;
     p->xbd_lds = 0;
     p->xbd_ct = nullptr;
+    p->xbd_mf = nullptr;
     p->xbd_nt = MR_XBD_THREADS;
     const mr_config &cfg = p->cfg;
     const int L = cfg.L, Mx = p->dims.m[0];
@@ -785,9 +788,27 @@ static int mr_xbd_setup(mr_aha_plan p)
         (void)hipGetLastError();
         return MR_SUCCESS;
     }
+    // The same kernel with its block step on the matrix cores: FP32 kernel,
+    // unmasked, L a multiple of 4 and an even kx centering shift (adjacent
+    // locations then have adjacent kernel entries). MR_AHA_HIP_MFMA=0 keeps
+    // the FMA walks.
+    mr_xbd_ct_fn mf = nullptr;
+    if (ct && !cfg.is_masked && p->kf == MR_KF_FP32 && L % 4 == 0 && (Mx / 2) % 2 == 0 &&
+        !p->no_mf && mr_env_ll("MR_AHA_HIP_MFMA", 1) != 0) {
+        int mgmax = 0;
+        mf = (nt > MR_XBD_THREADS) ? mr_xbd_ct_pick_wide(Mx, L, false, p->kf, true)
+                                   : mr_xbd_ct_pick(Mx, L, false, p->kf, &mgmax, true);
+        if (mf && lds > 65536 &&
+            hipFuncSetAttribute((const void *)mf, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds) != hipSuccess) {
+            (void)hipGetLastError();
+            mf = nullptr;
+        }
+    }
     p->xbd_g = (int)g;
     p->xbd_lds = lds;
     p->xbd_ct = ct;
+    p->xbd_mf = mf;
     p->xbd_nt = nt;
     p->nxdiv = mr_fdiv_make((unsigned)p->dims.n[0]);
     return MR_SUCCESS;
@@ -1007,7 +1028,11 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
                     blocks);
         MR_GRID_CHECK(blocks, p->xbd_ct ? p->xbd_nt : MR_XBD_THREADS, "fused x pass");
         if (p->xbd_ct) {
-            hipLaunchKernelGGL(p->xbd_ct, dim3((unsigned)blocks), dim3((unsigned)p->xbd_nt), p->xbd_lds,
+            // the matrix-core block step reads kernel entries two locations
+            // per 16-byte load
+            const mr_xbd_ct_fn fn =
+                (p->xbd_mf && ((unsigned long long)ker & 15) == 0) ? p->xbd_mf : p->xbd_ct;
+            hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3((unsigned)p->xbd_nt), p->xbd_lds,
                                s, B, ker, p->d_inv, kstride, d, p->fx.tws, R, G, nc, ncg, p->kq,
                                p->ksc);
             MR_LAUNCH_CHECK("mr_k_xbd_ct");
@@ -1259,6 +1284,20 @@ static int mr_autotune_impl(mr_aha_plan p, const mr_complex *x, const mr_complex
             }
         }
     }
+    // block step on the matrix cores against the FMA walks
+    if (rc == MR_SUCCESS && p->xbd_mf) {
+        double mf_ms = 0.0, fma_ms = 0.0;
+        rc = stage_ms(MR_ST_BD, &mf_ms);
+        if (rc == MR_SUCCESS) {
+            p->no_mf = 1;
+            rc = mr_xbd_setup(p);
+            if (rc == MR_SUCCESS) rc = stage_ms(MR_ST_BD, &fma_ms);
+            if (rc == MR_SUCCESS && fma_ms >= mf_ms) {
+                p->no_mf = 0;
+                rc = mr_xbd_setup(p);
+            }
+        }
+    }
     p->tuning = false;
     const int rc2 = zero_prof();
     return rc != MR_SUCCESS ? rc : rc2;
@@ -1291,9 +1330,9 @@ extern "C" MR_EXPORT int mr_aha_plan_tiles(mr_aha_plan plan, int *tiles, int n)
     mr_clear_error();
     MR_REQUIRE(plan != nullptr, "mr_aha_plan_tiles: plan is NULL");
     MR_REQUIRE(tiles != nullptr || n == 0, "mr_aha_plan_tiles: tiles is NULL");
-    const int v[5] = {plan->tile[0], plan->tile[1], plan->tile[2], plan->tile[3],
-                      plan->xbd_nt > MR_XBD_THREADS ? 1 : 0};
-    for (int i = 0; i < n && i < 5; ++i) tiles[i] = v[i];
+    const int v[6] = {plan->tile[0], plan->tile[1], plan->tile[2], plan->tile[3],
+                      plan->xbd_nt > MR_XBD_THREADS ? 1 : 0, plan->xbd_mf ? 1 : 0};
+    for (int i = 0; i < n && i < 6; ++i) tiles[i] = v[i];
     return MR_SUCCESS;
 }
 

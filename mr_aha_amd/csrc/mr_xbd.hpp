@@ -306,6 +306,193 @@ __device__ __forceinline__ void mr_bd_walk_pair_x2(float2 *a0, float2 *a1, int m
 }
 
 // ---------------------------------------------------------------------------
+// Block multiply on the matrix cores (v_mfma_f32_4x4x1_16b_f32)
+// ---------------------------------------------------------------------------
+// One wavefront multiplies 32 adjacent locations e0 .. e0+31 for a pair of
+// coils (or one coil). The instruction holds 16 independent 4x4 blocks; block
+// b is one location, so each block carries its own k-point's matrix:
+//   D_b[m][n] += A_b[m] * B_b[n]     (K = 1 per instruction, lane 4b+m / 4b+n)
+// Rows m are (coil, part) = (c0 re, c0 im, c1 re, c1 im) of the inputs v_k;
+// columns n are four outputs i = 4 ct + n. With A(k, i) = P + iQ the complex
+// row-vector product y_i = sum_k v_k A(k, i) is, per coil c,
+//   Re y_i = sum_k Re v_k P - Im v_k Q,   Im y_i = sum_k Re v_k Q + Im v_k P,
+// so two accumulators per column tile (v P and v Q, over k = 0..L-1) give
+// both coils' outputs in every lane's own registers: D1[2c] - D2[2c+1] and
+// D2[2c] + D1[2c+1].
+//
+// Lane 4b + n loads A(k, 4 ct + n) of locations e0 + 2b and e0 + 2b + 1 with
+// one 16-byte load (adjacent locations have adjacent kernel entries), so a
+// wavefront runs two sets of blocks, the even and the odd locations. A(k, i)
+// comes from the stored column i ("column form", k >= i lower / k <= i upper)
+// or, conjugated, from the stored column k ("row form"). The column tiles run
+// in two halves (fewer accumulators live); outputs of the first half are
+// held until the second half has read every input, then all are stored in
+// place.
+typedef float mr_f4 __attribute__((ext_vector_type(4)));
+#ifndef MR_MF_RING
+#define MR_MF_RING 4
+#endif
+
+template <int L, bool UPPER> struct mr_mf_units {
+    static constexpr int NCT = L / 4;                 // column tiles of 4 outputs
+    static constexpr int H = NCT >= 2 ? 2 : 1;        // halves
+    static constexpr int CPH = NCT / H;               // tiles per half
+    static constexpr int NU = H * L * CPH;            // 16-byte loads per lane
+    static constexpr int half(int u) { return u / (L * CPH); }
+    static constexpr int k(int u) { return (u / CPH) % L; }
+    static constexpr int cc(int u) { return u % CPH; }
+    static constexpr int ct(int u) { return half(u) * CPH + cc(u); }
+    // packed offset of stored column i's first row (column form: t = cs + k)
+    static constexpr int cs(int i) { return UPPER ? i * (i + 1) / 2 : i * L - i * (i - 1) / 2 - i; }
+    // row form: t = rs(k) + i
+    static constexpr int rs(int kk) { return UPPER ? kk * (kk + 1) / 2 : kk * L - kk * (kk - 1) / 2 - kk; }
+    // 0: every lane of unit u reads the column form, 1: every lane the row
+    // form, 2: lane n chooses (the unit crosses the diagonal)
+    static constexpr int form(int u)
+    {
+        const int kk = k(u), i0 = 4 * ct(u);
+        if (!UPPER) return kk >= i0 + 3 ? 0 : (kk < i0 ? 1 : 2);
+        return kk <= i0 ? 0 : (kk > i0 + 3 ? 1 : 2);
+    }
+};
+
+template <int L, bool UPPER, int U>
+__device__ __forceinline__ mr_f4 mr_mf_load(const float2 *const (&colp)[L / 4], const float2 *rowp,
+                                            long long ks, int n)
+{
+    using M = mr_mf_units<L, UPPER>;
+    constexpr int kk = M::k(U), ct = M::ct(U), f = M::form(U);
+    const float2 *pc = colp[ct] + (long long)kk * ks;
+    const float2 *pr = rowp + (long long)(M::rs(kk) + 4 * ct) * ks;
+    const float2 *p;
+    if constexpr (f == 0) {
+        p = pc;
+    } else if constexpr (f == 1) {
+        p = pr;
+    } else {
+        const bool col = UPPER ? (n >= kk - 4 * ct) : (n <= kk - 4 * ct);
+        p = col ? pc : pr;
+    }
+    return *reinterpret_cast<const mr_f4 *>(p);
+}
+
+template <int L, bool UPPER, int D, int U = 0>
+__device__ __forceinline__ void mr_mf_prefetch(mr_f4 (&ring)[D], const float2 *const (&colp)[L / 4],
+                                               const float2 *rowp, long long ks, int n)
+{
+    if constexpr (U < D && U < mr_mf_units<L, UPPER>::NU) {
+        ring[U] = mr_mf_load<L, UPPER, U>(colp, rowp, ks, n);
+        mr_mf_prefetch<L, UPPER, D, U + 1>(ring, colp, rowp, ks, n);
+    }
+}
+
+template <int L, bool UPPER, int D, int NC, int U>
+__device__ __forceinline__ void mr_mf_step(mr_f4 (&ring)[D], mr_f4 (&acc)[2][L / 4][2],
+                                           float2 (&keep)[2][L / 4][2], const float2 *const (&colp)[L / 4],
+                                           const float2 *rowp, long long ks, int n, const float *vb,
+                                           const float *vb2, int ms, float (&av)[2])
+{
+    using M = mr_mf_units<L, UPPER>;
+    if constexpr (U < M::NU) {
+        constexpr int kk = M::k(U), cc = M::cc(U), ct = M::ct(U), f = M::form(U);
+        if constexpr (cc == 0) {
+            // inputs v_k of this lane's row: (coil n >> 1, part n & 1) at the
+            // two locations of the lane's block; a second coil absent: zero.
+            // Rows from L/2 on are addressed from a second base (vb2), which
+            // keeps the LDS offsets immediate.
+            constexpr int kh = kk >= L / 2 ? kk - L / 2 : kk;
+            const float *v = kk >= L / 2 ? vb2 : vb;
+            av[0] = v[2 * kh * ms];
+            av[1] = v[2 * kh * ms + 2];
+            if constexpr (NC == 1) {
+                if (n >= 2) av[0] = av[1] = 0.f;
+            }
+        }
+        const mr_f4 q = ring[U % D];
+        if constexpr (U + D < M::NU) ring[U % D] = mr_mf_load<L, UPPER, U + D>(colp, rowp, ks, n);
+        float q0 = q.y, q1 = q.w;
+        if constexpr (f == 1) {
+            q0 = -q0;
+            q1 = -q1;
+        } else if constexpr (f == 2) {
+            const bool col = UPPER ? (n >= kk - 4 * ct) : (n <= kk - 4 * ct);
+            if (!col) {
+                q0 = -q0;
+                q1 = -q1;
+            }
+        }
+        acc[0][cc][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(av[0], q.x, acc[0][cc][0], 0, 0, 0);
+        acc[0][cc][1] = __builtin_amdgcn_mfma_f32_4x4x1f32(av[0], q0, acc[0][cc][1], 0, 0, 0);
+        acc[1][cc][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(av[1], q.z, acc[1][cc][0], 0, 0, 0);
+        acc[1][cc][1] = __builtin_amdgcn_mfma_f32_4x4x1f32(av[1], q1, acc[1][cc][1], 0, 0, 0);
+        if constexpr (kk == L - 1 && cc == M::CPH - 1) {
+            // end of a half: outputs of its column tiles, y = D1 - D2 / D2 + D1
+            constexpr int h = M::half(U);
+#pragma unroll
+            for (int c2 = 0; c2 < M::CPH; ++c2) {
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        const mr_f4 d1 = acc[s][c2][0], d2 = acc[s][c2][1];
+                        keep[s][h * M::CPH + c2][c] =
+                            make_float2(d1[2 * c] - d2[2 * c + 1], d2[2 * c] + d1[2 * c + 1]);
+                    }
+                    acc[s][c2][0] = mr_f4{0.f, 0.f, 0.f, 0.f};
+                    acc[s][c2][1] = mr_f4{0.f, 0.f, 0.f, 0.f};
+                }
+            }
+        }
+        mr_mf_step<L, UPPER, D, NC, U + 1>(ring, acc, keep, colp, rowp, ks, n, vb, vb2, ms, av);
+    }
+}
+
+// ap: the pair's first coil (pencil l at ap + l * ms, location e at + e); the
+// second coil at ap + L * ms. kb: kernel index of location e0 + 2b (even).
+template <int L, bool UPPER, int NC>
+__device__ __forceinline__ void mr_bd_mfma32(float2 *ap, int ms, const float2 *ker, long long ks,
+                                             long long kb, int e0, int lane)
+{
+    static_assert(L % 4 == 0, "MFMA block multiply: L a multiple of 4");
+    using M = mr_mf_units<L, UPPER>;
+    constexpr int D = M::NU < MR_MF_RING ? M::NU : MR_MF_RING;
+    // opaque per call: the per-lane kernel bases below are recomputed for each
+    // 32-location task instead of being hoisted out of the caller's loop for
+    // every instantiation at once (which spilled)
+    int n = lane & 3, b = lane >> 2;
+    asm volatile("" : "+v"(n), "+v"(b));
+    const int e = e0 + 2 * b;
+    const float2 *kbase = ker + kb;
+    const float2 *colp[M::NCT];
+#pragma unroll
+    for (int ct = 0; ct < M::NCT; ++ct) colp[ct] = kbase + (long long)M::cs(4 * ct + n) * ks;
+    const float2 *rowp = kbase + (long long)n * ks;
+    mr_f4 ring[D];
+    mr_mf_prefetch<L, UPPER, D>(ring, colp, rowp, ks, n);
+    const float *vb =
+        reinterpret_cast<const float *>(ap + (long long)((n >> 1) & (NC - 1)) * L * ms + e) + (n & 1);
+    const float *vb2 = vb + 2 * (L / 2) * ms;
+    asm volatile("" : "+v"(vb2));
+    mr_f4 acc[2][M::NCT][2];
+    float2 keep[2][M::NCT][2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int c = 0; c < M::NCT; ++c) acc[s][c][0] = acc[s][c][1] = mr_f4{0.f, 0.f, 0.f, 0.f};
+    float av[2];
+    mr_mf_step<L, UPPER, D, NC, 0>(ring, acc, keep, colp, rowp, ks, n, vb, vb2, ms, av);
+    // every input has been read: store both locations of each output at once
+#pragma unroll
+    for (int ct = 0; ct < M::NCT; ++ct)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const float2 y0 = keep[0][ct][c], y1 = keep[1][ct][c];
+            *reinterpret_cast<mr_f4 *>(ap + (long long)(c * L + 4 * ct + n) * ms + e) =
+                mr_f4{y0.x, y0.y, y1.x, y1.y};
+        }
+}
+
+// ---------------------------------------------------------------------------
 // 16-bit kernel format (BF16 or FP16 entries with FP32 scales per 4x4 block)
 // ---------------------------------------------------------------------------
 // Entry t of location k is stored as one 32-bit word q[t * kcol + k] (real
@@ -431,10 +618,12 @@ __device__ __forceinline__ void mr_bd_walk_q(float2 *a, int ms, const unsigned *
 typedef void (*mr_xbd_ct_fn)(float2 *, const float2 *, const int *, long long, mr_dims,
                              const float2 *, long long, int, int, int, const unsigned *,
                              const float *);
-mr_xbd_ct_fn mr_xbd_ct_pick(int n, int L, bool masked, int kf, int *gmax);
+// mf: the variant whose block step runs on the matrix cores (FP32 kernel,
+// unmasked; the caller checks the kernel's 16-byte alignment per launch).
+mr_xbd_ct_fn mr_xbd_ct_pick(int n, int L, bool masked, int kf, int *gmax, bool mf = false);
 // The workgroup of 1024 threads and two coils (160 KB of LDS, one workgroup
 // per CU) for x length n, rank L and kernel format kf, or nullptr.
-mr_xbd_ct_fn mr_xbd_ct_pick_wide(int n, int L, bool masked, int kf);
+mr_xbd_ct_fn mr_xbd_ct_pick_wide(int n, int L, bool masked, int kf, bool mf = false);
 
 // Fill the 16-bit format q [T, kcol], sc [NB, kcol] from an FP32 kernel.
 int mr_kf_quantize(int kf, const float2 *ker, long long kcol, int L, int upper, unsigned *q,

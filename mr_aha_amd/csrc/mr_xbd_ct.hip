@@ -95,8 +95,11 @@ __device__ __forceinline__ void mr_xbd_load_src2(float2 *Ap, __amdgpu_buffer_rsr
 // NT threads, up to GMAX coils per workgroup (the default: two workgroups of
 // 512 threads per CU; the wide form: 1024 threads and two coils in 160 KB,
 // one workgroup per CU). Either way a CU holds 16 wavefronts.
+// MF: the block step on the matrix cores (mr_bd_mfma32; FP32 kernel,
+// unmasked, 16-byte aligned kernel), a wavefront per 32 locations of a coil
+// pair.
 template <int N, int L, bool MASKED, int KF, int NT = MR_XBD_THREADS,
-          int GMAX = mr_xbd_ct_gmax(N, L)>
+          int GMAX = mr_xbd_ct_gmax(N, L), bool MF = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
 mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__restrict__ inv,
             long long kstride, mr_dims d, const float2 *tws, long long R, int G, int ncoil, int ncg,
@@ -157,7 +160,28 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
         if (kx >= N) kx -= N;
         return rowc + kx;
     };
-    if constexpr (KF == MR_KF_FP32 && GMAX >= 2) {
+    if constexpr (MF) {
+        static_assert(!MASKED && KF == MR_KF_FP32 && N % 32 == 0, "MFMA block step");
+        // a wavefront per 32 adjacent locations of a coil pair (an odd last
+        // coil alone); adjacent locations have adjacent kernel entries (N and
+        // the kx centering shift even), the kernel is 16-byte aligned
+        constexpr int NB = N / 32;
+        const int npair = (gn + 1) >> 1;
+        for (int task = wave; task < npair * NB; task += NW) {
+            const int qp = task / NB;
+            const int e0 = (task - qp * NB) * 32;
+            float2 *ap = A + 2 * qp * L * N;
+            const long long kb = kidx(e0 + 2 * (lane >> 2));
+            const bool two = 2 * qp + 1 < gn;
+            if (d.tri_upper) {
+                if (two) mr_bd_mfma32<L, true, 2>(ap, N, ker, kstride, kb, e0, lane);
+                else mr_bd_mfma32<L, true, 1>(ap, N, ker, kstride, kb, e0, lane);
+            } else {
+                if (two) mr_bd_mfma32<L, false, 2>(ap, N, ker, kstride, kb, e0, lane);
+                else mr_bd_mfma32<L, false, 1>(ap, N, ker, kstride, kb, e0, lane);
+            }
+        }
+    } else if constexpr (KF == MR_KF_FP32 && GMAX >= 2) {
         // coils in pairs: a thread walks one location for two coils, loading
         // each kernel entry once for both; an odd last coil walks alone (the
         // 16-bit kernel, half the bytes per entry, measured no gain from it)
@@ -269,12 +293,16 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
     }
 }
 
-template <int N> mr_xbd_ct_fn mr_xbd_ct_pick_n(int L, bool masked, int *gmax)
+template <int N> mr_xbd_ct_fn mr_xbd_ct_pick_n(int L, bool masked, int *gmax, bool mf)
 {
     switch (L) {
 #define MR_CT_CASE(l)                                                          \
     case l:                                                                    \
         *gmax = mr_xbd_ct_gmax(N, l);                                          \
+        if (mf)                                                                \
+            return masked ? nullptr                                            \
+                          : mr_k_xbd_ct<N, l, false, MR_KF_FP32, MR_XBD_THREADS, \
+                                        mr_xbd_ct_gmax(N, l), true>;           \
         return masked ? mr_k_xbd_ct<N, l, true, MR_KF_FP32>                    \
                       : mr_k_xbd_ct<N, l, false, MR_KF_FP32>;
         MR_CT_CASE(4) MR_CT_CASE(8) MR_CT_CASE(16)
@@ -345,11 +373,16 @@ __global__ void mr_k_quantize(const float2 *__restrict__ ker, long long kcol, in
 
 }  // namespace
 
-mr_xbd_ct_fn mr_xbd_ct_pick_wide(int n, int L, bool masked, int kf)
+mr_xbd_ct_fn mr_xbd_ct_pick_wide(int n, int L, bool masked, int kf, bool mf)
 {
     // FP32 only: the 16-bit walk moves half the bytes and measured slower in
     // the wide form (one workgroup per CU) than in two 80 KB workgroups
     if (L != 16 || (n != 640 && n != 512) || kf != MR_KF_FP32) return nullptr;
+    if (mf) {
+        if (masked) return nullptr;
+        return n == 640 ? mr_k_xbd_ct<640, 16, false, MR_KF_FP32, 1024, 2, true>
+                        : mr_k_xbd_ct<512, 16, false, MR_KF_FP32, 1024, 2, true>;
+    }
     if (n == 640)
         return masked ? mr_k_xbd_ct<640, 16, true, MR_KF_FP32, 1024, 2>
                       : mr_k_xbd_ct<640, 16, false, MR_KF_FP32, 1024, 2>;
@@ -357,17 +390,19 @@ mr_xbd_ct_fn mr_xbd_ct_pick_wide(int n, int L, bool masked, int kf)
                   : mr_k_xbd_ct<512, 16, false, MR_KF_FP32, 1024, 2>;
 }
 
-mr_xbd_ct_fn mr_xbd_ct_pick(int n, int L, bool masked, int kf, int *gmax)
+mr_xbd_ct_fn mr_xbd_ct_pick(int n, int L, bool masked, int kf, int *gmax, bool mf)
 {
     *gmax = 0;
-    if (kf == MR_KF_BF16) return masked ? nullptr : mr_xbd_ct_pick_q<MR_KF_BF16>(n, L, gmax);
-    if (kf == MR_KF_FP16) return masked ? nullptr : mr_xbd_ct_pick_q<MR_KF_FP16>(n, L, gmax);
+    if (kf == MR_KF_BF16)
+        return (masked || mf) ? nullptr : mr_xbd_ct_pick_q<MR_KF_BF16>(n, L, gmax);
+    if (kf == MR_KF_FP16)
+        return (masked || mf) ? nullptr : mr_xbd_ct_pick_q<MR_KF_FP16>(n, L, gmax);
     switch (n) {
-    case 256: return mr_xbd_ct_pick_n<256>(L, masked, gmax);
-    case 320: return mr_xbd_ct_pick_n<320>(L, masked, gmax);
-    case 384: return mr_xbd_ct_pick_n<384>(L, masked, gmax);
-    case 512: return mr_xbd_ct_pick_n<512>(L, masked, gmax);
-    case 640: return mr_xbd_ct_pick_n<640>(L, masked, gmax);
+    case 256: return mr_xbd_ct_pick_n<256>(L, masked, gmax, mf);
+    case 320: return mr_xbd_ct_pick_n<320>(L, masked, gmax, mf);
+    case 384: return mr_xbd_ct_pick_n<384>(L, masked, gmax, mf);
+    case 512: return mr_xbd_ct_pick_n<512>(L, masked, gmax, mf);
+    case 640: return mr_xbd_ct_pick_n<640>(L, masked, gmax, mf);
     default: return nullptr;
     }
 }
