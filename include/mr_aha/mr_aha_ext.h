@@ -97,6 +97,21 @@ MR_EXPORT int mr_op_set_kernel_format(mr_op op, int format);
 MR_EXPORT int mr_op_cg(mr_op op, mr_op op_refresh, const mr_complex *d_b, mr_complex *d_x,
                        double lambda, int n_iter, int refresh, int flags, double *res_hist);
 
+/* mr_op_cg without any allocation, for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, kernels and results as mr_op_cg; its work vectors (3 n complex
+ * values), reduction partials and residual history live in a static 64 MiB
+ * device array, so n and n_iter are limited to
+ * 3 * ceil256(8 n) + 8192 + 256 + ceil256(8 (n_iter + 2)) <= 64 MiB
+ * (about 2.7 M unknowns); larger problems fail with MR_ERR_INVALID_ARG.
+ * MR_CG_GRAPH is rejected (graph instantiation allocates). Without res_hist
+ * the call returns with the solve queued on the operator's stream. Every
+ * call on one device shares that device's array: calls that overlap on one
+ * GPU (other streams, other threads) race. No counterpart in the
+ * reference. */
+MR_EXPORT int mr_op_cg_noalloc(mr_op op, mr_op op_refresh, const mr_complex *d_b,
+                               mr_complex *d_x, double lambda, int n_iter, int refresh,
+                               int flags, double *res_hist);
 
 #ifdef __cplusplus
 }

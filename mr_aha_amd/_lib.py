@@ -196,6 +196,8 @@ _decl("mr_aha_plan_tiles", ctypes.c_int, [_p, ctypes.POINTER(ctypes.c_int), ctyp
 _decl("mr_op_tiles", ctypes.c_int, [_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int])
 _decl("mr_op_cg", ctypes.c_int, [_p, _p, _p, _p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, ctypes.POINTER(ctypes.c_double)])
+_decl("mr_op_cg_noalloc", ctypes.c_int, [_p, _p, _p, _p, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)])
 CG_ZERO_INIT = 1
 CG_GRAPH = 2
 _decl("mr_op_set_kernel_format", ctypes.c_int, [_p, ctypes.c_int])

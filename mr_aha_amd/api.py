@@ -528,8 +528,10 @@ class AhAOp:
         return dict(fo=t[0], fy=t[1], iy=t[2], io=t[3], wide=bool(t[4]), mfma=bool(t[5]))
 
     def cg(self, b, x=None, *, lam: float = 0.0, iters: int = 40, refresh: int = 10,
-           op_refresh: Optional["AhAOp"] = None, graph: bool = False):
-        """Conjugate gradients on (A^H A + lam I) x = b (library entry mr_op_cg).
+           op_refresh: Optional["AhAOp"] = None, graph: bool = False, noalloc: bool = False):
+        """Conjugate gradients on (A^H A + lam I) x = b (library entry mr_op_cg;
+        ``noalloc``: mr_op_cg_noalloc, the same solve on the library's static
+        work array, without graphs).
 
         FP32 vectors, FP64 inner products and step lengths; every ``refresh``
         iterations the residual is recomputed as b - (A^H A + lam I) x with
@@ -550,8 +552,10 @@ class AhAOp:
             _Arg.numel(x, n, "x")
         ref = op_refresh._open() if op_refresh is not None else None
         hist = (ctypes.c_double * (int(iters) + 1))()
-        check(lib.mr_op_cg(h, ref, _Arg.ptr(b, "b", "complex64"), _Arg.ptr(x, "x", "complex64"),
-                           float(lam), int(iters), int(refresh), flags, hist), "mr_op_cg")
+        fn, name = (lib.mr_op_cg_noalloc, "mr_op_cg_noalloc") if noalloc else (lib.mr_op_cg,
+                                                                                "mr_op_cg")
+        check(fn(h, ref, _Arg.ptr(b, "b", "complex64"), _Arg.ptr(x, "x", "complex64"),
+                 float(lam), int(iters), int(refresh), flags, hist), name)
         return x, list(hist)
 
     def close(self) -> None:

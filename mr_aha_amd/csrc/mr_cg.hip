@@ -19,7 +19,7 @@
 #include "mr_aha/mr_aha_ext.h"
 
 #include <cmath>
-#include <vector>
+#include <mutex>
 
 int mr_op_stream_len(mr_op op, hipStream_t *s, long long *n, int *device);
 
@@ -267,41 +267,46 @@ int cg_capture(mr_op op, mr_op opr, hipStream_t s, const cg_ws &w, const float2 
     return MR_SUCCESS;
 }
 
-}  // namespace
-
-extern "C" MR_EXPORT int mr_op_cg(mr_op op, mr_op op_refresh, const mr_complex *d_b,
-                                  mr_complex *d_x, double lambda, int n_iter, int refresh,
-                                  int flags, double *res_hist)
+// Checks shared by both entry points; fills the stream, length and device.
+int cg_args(const char *fn, mr_op op, mr_op opr, const mr_complex *d_b, const mr_complex *d_x,
+            double lambda, int n_iter, int refresh, int flags, hipStream_t *s, long long *n,
+            int *dev)
 {
-    mr_clear_error();
-    MR_REQUIRE(op != nullptr, "mr_op_cg: op is NULL");
-    MR_REQUIRE(d_b != nullptr && d_x != nullptr, "mr_op_cg: d_b and d_x are required");
-    MR_REQUIRE((const void *)d_b != (const void *)d_x, "mr_op_cg: d_x must not alias d_b");
-    MR_REQUIRE(n_iter >= 0, "mr_op_cg: n_iter = %d must be >= 0", n_iter);
-    MR_REQUIRE(refresh >= 0, "mr_op_cg: refresh = %d must be >= 0", refresh);
-    MR_REQUIRE(lambda >= 0.0, "mr_op_cg: lambda = %g must be >= 0", lambda);
-    mr_op opr = op_refresh ? op_refresh : op;
-    hipStream_t s = nullptr, s2 = nullptr;
-    long long n = 0, n2 = 0;
-    int dev = 0, dev2 = 0;
-    MR_TRY(mr_op_stream_len(op, &s, &n, &dev));
+    MR_REQUIRE(op != nullptr, "%s: op is NULL", fn);
+    MR_REQUIRE(d_b != nullptr && d_x != nullptr, "%s: d_b and d_x are required", fn);
+    MR_REQUIRE((const void *)d_b != (const void *)d_x, "%s: d_x must not alias d_b", fn);
+    MR_REQUIRE(n_iter >= 0, "%s: n_iter = %d must be >= 0", fn, n_iter);
+    MR_REQUIRE(refresh >= 0, "%s: refresh = %d must be >= 0", fn, refresh);
+    MR_REQUIRE(lambda >= 0.0, "%s: lambda = %g must be >= 0", fn, lambda);
+    hipStream_t s2 = nullptr;
+    long long n2 = 0;
+    int dev2 = 0;
+    MR_TRY(mr_op_stream_len(op, s, n, dev));
     MR_TRY(mr_op_stream_len(opr, &s2, &n2, &dev2));
-    MR_REQUIRE(n == n2 && dev == dev2 && s == s2,
-               "mr_op_cg: op_refresh must have the same shape, device and stream as op");
-    const bool graph = (flags & MR_CG_GRAPH) != 0;
-    if (graph) {
-        MR_REQUIRE(s != nullptr, "mr_op_cg: graph capture needs a non-default cfg.stream");
+    MR_REQUIRE(*n == n2 && *dev == dev2 && *s == s2,
+               "%s: op_refresh must have the same shape, device and stream as op", fn);
+    if (flags & MR_CG_GRAPH) {
+        MR_REQUIRE(*s != nullptr, "%s: graph capture needs a non-default cfg.stream", fn);
         MR_REQUIRE(mr_env_ll("MR_AHA_HIP_PROFILE", 0) == 0,
-                   "mr_op_cg: graph capture is incompatible with MR_AHA_HIP_PROFILE");
+                   "%s: graph capture is incompatible with MR_AHA_HIP_PROFILE", fn);
     }
-    int prev = 0;
-    MR_HIP_CHECK(hipGetDevice(&prev));
-    if (prev != dev) MR_HIP_CHECK(hipSetDevice(dev));
+    return MR_SUCCESS;
+}
+
+// The solve on work vectors w (allocated by the caller of this function).
+// Runs on the operator's device, which must be current.
+// sync_end: wait for the stream before returning (the work vectors are freed
+// afterwards); otherwise a call without res_hist returns with the solve
+// queued on the stream.
+int cg_run(mr_op op, mr_op opr, hipStream_t s, const cg_ws &w, const mr_complex *d_b,
+           mr_complex *d_x, double lambda, int n_iter, int refresh, int flags, long long n,
+           double *res_hist, bool sync_end)
+{
+    const bool graph = (flags & MR_CG_GRAPH) != 0;
     const float2 *b = reinterpret_cast<const float2 *>(d_b);
     float2 *x = reinterpret_cast<float2 *>(d_x);
     const float lam = (float)lambda;
     const unsigned G = grid_for(n);
-    cg_ws w;
     hipGraphExec_t gx[2] = {nullptr, nullptr};
     int rc = MR_SUCCESS;
     auto fail = [&](int code) {
@@ -309,17 +314,6 @@ extern "C" MR_EXPORT int mr_op_cg(mr_op op, mr_op op_refresh, const mr_complex *
         return code;
     };
     do {
-        const size_t vb = (size_t)n * sizeof(float2);
-        if (hipMalloc((void **)&w.r, vb) != hipSuccess || hipMalloc((void **)&w.p, vb) != hipSuccess ||
-            hipMalloc((void **)&w.Ap, vb) != hipSuccess ||
-            hipMalloc((void **)&w.part, CG_BLOCKS * sizeof(double2)) != hipSuccess ||
-            hipMalloc((void **)&w.scal, S_N * sizeof(double)) != hipSuccess ||
-            hipMalloc((void **)&w.hist, (size_t)(n_iter + 2) * sizeof(double)) != hipSuccess) {
-            (void)hipGetLastError();
-            mr_set_error("mr_op_cg: failed to allocate %zu bytes of work vectors", 3 * vb);
-            fail(MR_ERR_ALLOC);
-            break;
-        }
         // |b|^2; r0 = b - (A^H A + lambda) x0, or b when x0 = 0
         hipLaunchKernelGGL(k_dot, dim3(G), dim3(CG_THREADS), 0, s, b, b, n, w.part);
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(CG_THREADS), 0, s, w.part, (int)G, w.scal,
@@ -373,23 +367,25 @@ extern "C" MR_EXPORT int mr_op_cg(mr_op op, mr_op op_refresh, const mr_complex *
         }
         if (rc != MR_SUCCESS) break;
         if (res_hist) {
-            std::vector<double> h(n_iter + 1), sc(S_N);
-            if (hipMemcpyAsync(h.data(), w.hist, h.size() * sizeof(double), hipMemcpyDeviceToHost,
-                               s) != hipSuccess ||
-                hipMemcpyAsync(sc.data(), w.scal, S_N * sizeof(double), hipMemcpyDeviceToHost, s) !=
+            // squared residual norms straight into the caller's array, then
+            // scaled in place by |b|
+            double bb = 0.0;
+            if (hipMemcpyAsync(res_hist, w.hist, (size_t)(n_iter + 1) * sizeof(double),
+                               hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipMemcpyAsync(&bb, w.scal + S_BB, sizeof(double), hipMemcpyDeviceToHost, s) !=
                     hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess) {
                 mr_set_error("mr_op_cg: reading the residual history failed");
                 fail(MR_ERR_CUDA);
                 break;
             }
-            const double bb = sc[S_BB] > 0.0 ? sc[S_BB] : 1.0;
-            for (int k = 0; k <= n_iter; ++k) res_hist[k] = std::sqrt(h[k] / bb);
+            if (!(bb > 0.0)) bb = 1.0;
+            for (int k = 0; k <= n_iter; ++k) res_hist[k] = std::sqrt(res_hist[k] / bb);
         }
     } while (false);
     if (gx[0]) (void)hipGraphExecDestroy(gx[0]);
     if (gx[1]) (void)hipGraphExecDestroy(gx[1]);
-    if (rc == MR_SUCCESS && !res_hist) {
+    if (rc == MR_SUCCESS && !res_hist && sync_end) {
         if (hipStreamSynchronize(s) != hipSuccess) {
             mr_set_error("mr_op_cg: stream synchronisation failed");
             rc = MR_ERR_CUDA;
@@ -397,7 +393,118 @@ extern "C" MR_EXPORT int mr_op_cg(mr_op op, mr_op op_refresh, const mr_complex *
     } else if (rc != MR_SUCCESS) {
         (void)hipStreamSynchronize(s);
     }
+    return rc;
+}
+
+// Work memory of mr_op_cg_noalloc: r, p, Ap (n complex values each), the
+// reduction partials, the scalars and the residual history, carved from one
+// static device array (each device the library runs on holds its own copy
+// until the process ends). The host may not use the array's name as a
+// device address: its address on each device comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr size_t MR_CG_SCRATCH_BYTES = size_t(64) << 20;
+constexpr int MR_CG_MAX_DEVICES = 64;
+__device__ __align__(256) unsigned char scratch[MR_CG_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MR_CG_MAX_DEVICES] = {};
+
+int scratch_on_device(int dev, unsigned char **out)
+{
+    MR_REQUIRE(dev >= 0 && dev < MR_CG_MAX_DEVICES,
+               "mr_op_cg_noalloc: device %d is outside 0..%d", dev, MR_CG_MAX_DEVICES - 1);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            MR_FAIL(MR_ERR_CUDA, "mr_op_cg_noalloc: hipGetSymbolAddress of the scratch array "
+                                 "failed on device %d: %s", dev, hipGetErrorString(e));
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = static_cast<unsigned char *>(g_scratch_ptr[dev]);
+    return MR_SUCCESS;
+}
+
+size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace
+
+extern "C" MR_EXPORT int mr_op_cg(mr_op op, mr_op op_refresh, const mr_complex *d_b,
+                                  mr_complex *d_x, double lambda, int n_iter, int refresh,
+                                  int flags, double *res_hist)
+{
+    mr_clear_error();
+    mr_op opr = op_refresh ? op_refresh : op;
+    hipStream_t s = nullptr;
+    long long n = 0;
+    int dev = 0;
+    MR_TRY(cg_args("mr_op_cg", op, opr, d_b, d_x, lambda, n_iter, refresh, flags, &s, &n, &dev));
+    int prev = 0;
+    MR_HIP_CHECK(hipGetDevice(&prev));
+    if (prev != dev) MR_HIP_CHECK(hipSetDevice(dev));
+    cg_ws w;
+    int rc = MR_SUCCESS;
+    const size_t vb = (size_t)n * sizeof(float2);
+    if (hipMalloc((void **)&w.r, vb) != hipSuccess || hipMalloc((void **)&w.p, vb) != hipSuccess ||
+        hipMalloc((void **)&w.Ap, vb) != hipSuccess ||
+        hipMalloc((void **)&w.part, CG_BLOCKS * sizeof(double2)) != hipSuccess ||
+        hipMalloc((void **)&w.scal, S_N * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&w.hist, (size_t)(n_iter + 2) * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        mr_set_error("mr_op_cg: failed to allocate %zu bytes of work vectors", 3 * vb);
+        rc = MR_ERR_ALLOC;
+    } else {
+        rc = cg_run(op, opr, s, w, d_b, d_x, lambda, n_iter, refresh, flags, n, res_hist, true);
+    }
     w.free_all();
+    if (prev != dev) (void)hipSetDevice(prev);
+    return rc;
+}
+
+extern "C" MR_EXPORT int mr_op_cg_noalloc(mr_op op, mr_op op_refresh, const mr_complex *d_b,
+                                          mr_complex *d_x, double lambda, int n_iter,
+                                          int refresh, int flags, double *res_hist)
+{
+    mr_clear_error();
+    mr_op opr = op_refresh ? op_refresh : op;
+    hipStream_t s = nullptr;
+    long long n = 0;
+    int dev = 0;
+    MR_TRY(cg_args("mr_op_cg_noalloc", op, opr, d_b, d_x, lambda, n_iter, refresh, flags, &s, &n,
+                   &dev));
+    // graph instantiation allocates: this entry point runs its iterations eagerly
+    MR_REQUIRE((flags & MR_CG_GRAPH) == 0,
+               "mr_op_cg_noalloc: MR_CG_GRAPH is not available without allocation");
+    const size_t vb = round256((size_t)n * sizeof(float2));
+    const size_t need = 3 * vb + round256(CG_BLOCKS * sizeof(double2)) +
+                        round256(S_N * sizeof(double)) +
+                        round256((size_t)(n_iter + 2) * sizeof(double));
+    MR_REQUIRE(need <= MR_CG_SCRATCH_BYTES,
+               "mr_op_cg_noalloc: %lld unknowns and %d iterations need %zu bytes of work "
+               "memory, above the %zu-byte scratch array (use mr_op_cg)",
+               n, n_iter, need, MR_CG_SCRATCH_BYTES);
+    int prev = 0;
+    MR_HIP_CHECK(hipGetDevice(&prev));
+    if (prev != dev) MR_HIP_CHECK(hipSetDevice(dev));
+    unsigned char *base = nullptr;
+    int rc = scratch_on_device(dev, &base);
+    if (rc == MR_SUCCESS) {
+        cg_ws w;
+        w.r = reinterpret_cast<float2 *>(base);
+        w.p = reinterpret_cast<float2 *>(base + vb);
+        w.Ap = reinterpret_cast<float2 *>(base + 2 * vb);
+        unsigned char *q = base + 3 * vb;
+        w.part = reinterpret_cast<double2 *>(q);
+        q += round256(CG_BLOCKS * sizeof(double2));
+        w.scal = reinterpret_cast<double *>(q);
+        q += round256(S_N * sizeof(double));
+        w.hist = reinterpret_cast<double *>(q);
+        rc = cg_run(op, opr, s, w, d_b, d_x, lambda, n_iter, refresh, flags, n, res_hist,
+                    false);
+    }
     if (prev != dev) (void)hipSetDevice(prev);
     return rc;
 }

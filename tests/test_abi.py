@@ -26,9 +26,10 @@ def declared_symbols():
 
 def test_exports_every_declared_symbol(L):
     syms = declared_symbols()
-    # 25 reference entry points + mr_version + 6 extensions (profile, pipeline, CG,
-    # kernel format)
-    assert len(syms) == 36
+    # 25 reference entry points + mr_version + 11 extensions (profiling,
+    # pipeline and tiling queries, autotuner, kernel format, CG and its
+    # allocation-free variant)
+    assert len(syms) == 37
     for s in syms:
         assert hasattr(L.lib, s), s
     out = subprocess.run(["nm", "-D", "--defined-only", str(L.LIB_PATH)], capture_output=True,
