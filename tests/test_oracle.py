@@ -83,6 +83,53 @@ def test_cpu_baseline_matches_apply():
             assert rel(y, ref) < 1e-5, (name, up)
 
 
+@pytest.mark.parametrize("case", ["2d_50x40_L3_c3", "2d_33x27_L1_c2_odd", "3d_24x20x16_L4_c2",
+                                  "3d_25x21x15_L3_c2_odd", "stack_40x36x10_L3_c3",
+                                  "stack_31x36x9_L4_c2_odd"])
+def test_cpu_nufft_matches_nudft(case):
+    """The CPU NUFFT A^H A (ES kernel, upsampling 2; the benchmark's CPU
+    baseline) against the exact direct NUDFT normal operator: complex128 at
+    eps 1e-12 to 1e-10, complex64 at eps 1e-6 (the reference's default) to
+    1e-5; odd grids, 2D, 3D and stack."""
+    from oracle import cpu_nufft
+    g, traj, phi2, x, S = gen.reference_case(case)
+    ref = nudft.aha_nudft(g, x, S, phi2, traj)
+    y64 = cpu_nufft.aha_nufft(g, x, S, phi2, traj, eps=1e-12, dtype=np.complex128, threads=4)
+    y32 = cpu_nufft.aha_nufft(g, x, S, phi2, traj, eps=1e-6, dtype=np.complex64, threads=4)
+    assert rel(y64, ref) < 1e-10, rel(y64, ref)
+    assert rel(y32, ref) < 1e-5, rel(y32, ref)
+
+
+def test_cpu_nufft_planted_mistakes_fail():
+    """A conjugated Gram block, a flipped trajectory, a dropped kernel
+    deconvolution each move the CPU NUFFT far from the exact operator."""
+    from oracle import cpu_nufft
+    g, traj, phi2, x, S = gen.reference_case("stack_31x36x9_L4_c2_odd")
+    ref = nudft.aha_nudft(g, x, S, phi2, traj)
+    kw = dict(eps=1e-9, dtype=np.complex128, threads=2)
+    assert rel(cpu_nufft.aha_nufft(g, x, S, np.conj(phi2), traj, **kw), ref) > 1e-2
+    assert rel(cpu_nufft.aha_nufft(g, x, S, phi2, -traj, **kw), ref) > 1e-2
+    orig = cpu_nufft.es_fourier
+    try:
+        cpu_nufft.es_fourier = lambda w, beta, nf, N: np.ones(N) * orig(w, beta, nf, N)[N // 2]
+        assert rel(cpu_nufft.aha_nufft(g, x, S, phi2, traj, **kw), ref) > 1e-2
+    finally:
+        cpu_nufft.es_fourier = orig
+
+
+def test_cpu_nufft_plane_sample_is_partial_sum():
+    """A stack run restricted to kz planes (the baseline's timing sample) sums
+    over the planes to the full result."""
+    from oracle import cpu_nufft
+    g, traj, phi2, x, S = gen.reference_case("stack_40x36x10_L3_c3")
+    kw = dict(eps=1e-9, dtype=np.complex128, threads=2)
+    full = cpu_nufft.aha_nufft(g, x, S, phi2, traj, **kw)
+    nz = g.n[2]
+    parts = cpu_nufft.aha_nufft(g, x, S, phi2, traj, planes=range(0, nz // 2), **kw) + \
+        cpu_nufft.aha_nufft(g, x, S, phi2, traj, planes=range(nz // 2, nz), **kw)
+    assert rel(parts, full) < 1e-12
+
+
 @pytest.mark.parametrize("M", [4, 7, 64])
 def test_dirichlet_closed_form(M):
     # D_M(t) = sum_{d=-floor(M/2)}^{ceil(M/2)-1} e^{i d t}
