@@ -83,6 +83,20 @@ def test_struct_layouts(tmp_path, L):
             assert getattr(cls, field).offset == off, key
 
 
+def test_layout_equals_reference_header(tmp_path):
+    """Every struct's size and alignment, every field's offset and size and
+    every enumerator / MR_ constant of this header equal the reference
+    header's (built from the reference checkout with gcc by
+    oracle/abi_layout.py, kept in oracle/_ref/abi_ref.txt)."""
+    from oracle import abi_layout
+    ref = abi_layout.reference_layout()
+    if ref is None:
+        pytest.skip("neither the reference header nor oracle/_ref/abi_ref.txt is present")
+    ours = abi_layout.run(ROOT / "include", tmp_path)
+    assert len(ours.splitlines()) > 70
+    assert ours.splitlines() == ref.splitlines()
+
+
 def test_identity_and_status(L):
     assert L.backend() == "cuda"
     assert L.lib.mr_version() == 10000
