@@ -1,4 +1,4 @@
-"""Synthetic benchmark workloads generated directly on the GPU (torch).
+"""Synthetic benchmark workloads.
 
 Shapes of the benchmark configurations: golden-angle radial spokes (2D and
 stack-of-stars, the same spokes on every kz plane) or Fibonacci-sphere
@@ -7,12 +7,19 @@ complex Gaussian matrix; every readout (and kz plane) acquired at one frame
 with the rank-1 Gram block Phi(:,t) Phi(:,t)^H; smooth Gaussian-bump coil
 maps normalised to unit root-sum-of-squares; complex Gaussian
 coefficients x. All arrays use the binding layout of ``mr_aha_amd.api``.
+
+The random draws (the basis, then x) come from numpy's default_rng(seed) in
+a fixed order, so a workload equals the committed golden fixtures built
+from the same recipe and seed; the trajectories and coil maps are closed
+forms evaluated in float64 on the device. The tests check both against the
+fixtures' generator.
 """
 
 from __future__ import annotations
 
 import math
 
+import numpy as np
 import torch
 
 GOLDEN = (math.sqrt(5.0) - 1.0) / 2.0
@@ -38,12 +45,19 @@ def kooshball_traj(n_spokes, nkx, device):
         .to(torch.float32).contiguous()
 
 
-def basis(n_frames, L, seed):
-    g = torch.Generator().manual_seed(seed)
-    G = torch.randn(n_frames, L, generator=g, dtype=torch.float64) + \
-        1j * torch.randn(n_frames, L, generator=g, dtype=torch.float64)
-    Q, _ = torch.linalg.qr(G)
-    return Q.T.contiguous()  # (L, n_frames), orthonormal rows
+def basis_and_x(n_frames, L, seed, x_shape=None):
+    """Phi (L, n_frames): Q^T of the QR of a complex Gaussian (n_frames, L)
+    matrix (orthonormal rows), then x of ``x_shape`` (complex64), both from
+    numpy's default_rng(seed) in that order."""
+    rng = np.random.default_rng(seed)
+    G = rng.standard_normal((n_frames, L)) + 1j * rng.standard_normal((n_frames, L))
+    Q, _ = np.linalg.qr(G)
+    Phi = torch.from_numpy(Q.T.copy())
+    x = None
+    if x_shape is not None:
+        x = (rng.standard_normal(x_shape) + 1j * rng.standard_normal(x_shape)).astype(np.complex64)
+        x = torch.from_numpy(x)
+    return Phi, x
 
 
 def rank1_phi2(Phi, frames):
@@ -54,7 +68,7 @@ def rank1_phi2(Phi, frames):
 
 
 def coil_maps(img_shape, n_coils, three_d, device):
-    axes = [torch.arange(s, device=device, dtype=torch.float32) - s // 2 for s in img_shape]
+    axes = [torch.arange(s, device=device, dtype=torch.float64) - s // 2 for s in img_shape]
     grids = torch.meshgrid(*axes, indexing="ij")
     if len(img_shape) == 2:
         y, x = grids
@@ -91,36 +105,37 @@ def complex_gaussian(shape, seed, device):
 
 
 def stack_of_stars(n=320, nz=56, L=16, C=12, n_spokes=400, nkx=640, n_frames=200,
-                   seed=20261203, device="cuda"):
+                   seed=20261203, device="cuda", with_x=True):
     """320x320x56 image, in-plane grid 640x640, L=16, C=12, 400 spokes x 640
     samples on every kz plane, frame (spoke + 400 kz) mod 200 by default."""
     traj = radial_traj(n_spokes, nkx, device)
-    Phi = basis(n_frames, L, seed).to(device)
+    Phi, x = basis_and_x(n_frames, L, seed, (L, nz, n, n) if with_x else None)
+    Phi = Phi.to(device)
     frames = (torch.arange(n_spokes, device=device)[None, :] +
               n_spokes * torch.arange(nz, device=device)[:, None]) % n_frames
     phi2 = rank1_phi2(Phi, frames)
     S = coil_maps((nz, n, n), C, False, device)
-    x = complex_gaussian((L, nz, n, n), seed + 1, device)
     return dict(traj_type="stack", n_dim=(n, n, nz), L=L, n_coils=C, traj=traj, phi2=phi2,
-                SEs=S, x=x)
+                SEs=S, x=x.to(device) if with_x else None)
 
 
-def radial_2d(n=128, L=4, C=4, n_spokes=64, nkx=256, n_frames=64, seed=20261202, device="cuda"):
+def radial_2d(n=128, L=4, C=4, n_spokes=64, nkx=256, n_frames=64, seed=20261202, device="cuda",
+              with_x=True):
     """128x128 image, L=4, C=4, 64 golden-angle spokes x 256 samples by default."""
     traj = radial_traj(n_spokes, nkx, device)
-    Phi = basis(n_frames, L, seed).to(device)
-    phi2 = rank1_phi2(Phi, torch.arange(n_spokes, device=device) % n_frames)
+    Phi, x = basis_and_x(n_frames, L, seed, (L, n, n) if with_x else None)
+    phi2 = rank1_phi2(Phi.to(device), torch.arange(n_spokes, device=device) % n_frames)
     S = coil_maps((n, n), C, False, device)
-    x = complex_gaussian((L, n, n), seed + 1, device)
-    return dict(traj_type="2D", n_dim=(n, n), L=L, n_coils=C, traj=traj, phi2=phi2, SEs=S, x=x)
+    return dict(traj_type="2D", n_dim=(n, n), L=L, n_coils=C, traj=traj, phi2=phi2, SEs=S,
+                x=x.to(device) if with_x else None)
 
 
 def kooshball(n=160, L=8, C=8, n_spokes=20000, nkx=320, n_frames=100, seed=20261204,
-              device="cuda"):
+              device="cuda", with_x=True):
     """160^3 image, grid 320^3, L=8, C=8, 20000 Fibonacci-sphere spokes x 320."""
     traj = kooshball_traj(n_spokes, nkx, device)
-    Phi = basis(n_frames, L, seed).to(device)
-    phi2 = rank1_phi2(Phi, torch.arange(n_spokes, device=device) % n_frames)
+    Phi, x = basis_and_x(n_frames, L, seed, (L, n, n, n) if with_x else None)
+    phi2 = rank1_phi2(Phi.to(device), torch.arange(n_spokes, device=device) % n_frames)
     S = coil_maps((n, n, n), C, True, device)
-    x = complex_gaussian((L, n, n, n), seed + 1, device)
-    return dict(traj_type="3D", n_dim=(n, n, n), L=L, n_coils=C, traj=traj, phi2=phi2, SEs=S, x=x)
+    return dict(traj_type="3D", n_dim=(n, n, n), L=L, n_coils=C, traj=traj, phi2=phi2, SEs=S,
+                x=x.to(device) if with_x else None)
