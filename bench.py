@@ -121,140 +121,274 @@ def measured_traffic(kernel, workload, precision="fp32"):
     return None
 
 
-def cpu_baseline(w, K, cfg, budget_s=30.0):
-    """Oracle CPU Toeplitz apply on one coil of the workload (complex64)."""
+def host_threads() -> int:
+    """Host threads for the CPU legs: the process's CPU affinity, capped by
+    OMP_NUM_THREADS when the environment sets it (the share of the host this
+    process is given)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, int(n))
+
+
+def rel_err(a, b) -> float:
     import numpy as np
-    from oracle import cpu_baseline as cb
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(int(threads), 16))
+    a = np.asarray(a, dtype=np.complex128).ravel()
+    b = np.asarray(b, dtype=np.complex128).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def cpu_leg(w, coil_outputs, threads):
+    """CPU baseline and FP64 oracle of the workload, one coil.
+
+    Timed: the reference's CPU path, the NUFFT A^H A (ES kernel, eps 1e-6,
+    complex64; oracle/cpu_nufft.py) restricted to coil 0, all ranks and kz
+    planes, scaled by the coil count to applies/s. Oracle: the same operator
+    in complex128 at eps 1e-9 for coil 0, against which the GPU's one-coil
+    outputs (`coil_outputs`: name -> array) are stated."""
+    import numpy as np
+    from oracle import cpu_nufft
+    from oracle.conventions import make_geometry
+    g = make_geometry(w["traj_type"], w["n_dim"], w["L"], w["n_coils"])
     x = w["x"].cpu().numpy()
     S = w["SEs"][:1].cpu().numpy()
-    Kh = K.cpu().numpy()
-    n = list(cfg.n_dim)
-    M = list(cfg.phi2_dim)
+    phi2 = w["phi2"].cpu().numpy()
+    traj = w["traj"].cpu().numpy()
+    cpu_nufft.lib()
     t0 = time.perf_counter()
-    cb.toeplitz_apply_coils(n, M, cfg.L, x, S, Kh, cfg.triangulation == 1, [0], threads)
+    y32 = cpu_nufft.aha_nufft(g, x, S, phi2, traj, eps=1e-6, dtype=np.complex64,
+                              threads=threads)
     dt = time.perf_counter() - t0
-    del Kh
-    C = cfg.n_coils
-    return {"value": 1.0 / (dt * C), "unit": "A^H A applies/s", "cores": threads, "kind": "port",
-            "sample": f"1 of {C} coils of one apply (all {cfg.L} ranks, full grid), "
-                      f"{dt:.1f} s, scaled x{C}"}
+    t1 = time.perf_counter()
+    y64 = cpu_nufft.aha_nufft(g, x, S, phi2, traj, eps=1e-9, dtype=np.complex128,
+                              threads=threads)
+    dt64 = time.perf_counter() - t1
+    C = w["n_coils"]
+    base = {"value": 1.0 / (dt * C), "unit": "A^H A applies/s", "cores": threads, "kind": "port",
+            "sample": f"NUFFT A^H A (method=\"nufft\", ES kernel, eps 1e-6, complex64) of coil "
+                      f"0 of {C}, all ranks and planes: {dt:.1f} s on {threads} threads, "
+                      f"scaled x{C}",
+            "error_vs_fp64": rel_err(y32, y64)}
+    errs = {k: rel_err(v, y64) for k, v in coil_outputs.items()}
+    return base, errs, {"fp64_oracle": "NUFFT A^H A, complex128, eps 1e-9, coil 0",
+                        "fp64_oracle_s": round(dt64, 1)}
 
 
-def bench_cg(args):
-    """CG reconstructions of the 128x128 L=4 C=4 radial problem: 40 iterations,
-    true residual every 10, FP32 operator vs 16-bit kernel refined against it,
-    HIP-graph-replayed iterations. Error of each against the FP32 solution
-    converged to a 1e-6 relative residual (200 iterations)."""
+def smooth_field(shape, seed, device, frac=0.1):
+    """Complex Gaussian field low-passed to |f| < frac per in-plane axis."""
+    import torch
+    from mr_aha_amd import workloads
+    z = workloads.complex_gaussian(shape, seed, device)
+    f = torch.fft.fftn(z, dim=(-2, -1))
+    ky = torch.fft.fftfreq(shape[-2], device=device).abs()[:, None] < frac
+    kx = torch.fft.fftfreq(shape[-1], device=device).abs()[None, :] < frac
+    return torch.fft.ifftn(f * (ky & kx), dim=(-2, -1)).to(torch.complex64).contiguous()
+
+
+def power_lmax(op, shape, device, iters=20, v=None):
+    """Largest eigenvalue of A^H A by power iteration on the device (from v,
+    default a complex Gaussian)."""
+    import torch
+    from mr_aha_amd import workloads
+    if v is None:
+        v = workloads.complex_gaussian(shape, 4242, device)
+    lam = 0.0
+    for _ in range(iters):
+        v = v / torch.linalg.vector_norm(v)
+        u = op(v)
+        lam = float(torch.linalg.vector_norm(u))
+        v = u
+    return lam
+
+
+def cg_rhs_fp64(w, xt, lam, threads):
+    """Oracle side of the CG leg: b = (A^H A + lam I) x_true with the FP64
+    NUFFT operator (complex128, eps 1e-12), so that x_true is the FP64
+    solution the GPU solves are measured against."""
+    import numpy as np
+    from oracle import cpu_nufft
+    from oracle.conventions import make_geometry
+    g = make_geometry(w["traj_type"], w["n_dim"], w["L"], w["n_coils"])
+    x = xt.cpu().numpy()
+    y = cpu_nufft.aha_nufft(g, x, w["SEs"].cpu().numpy(), w["phi2"].cpu().numpy(),
+                            w["traj"].cpu().numpy(), eps=1e-12, dtype=np.complex128,
+                            threads=threads)
+    return (y + lam * x).astype(np.complex64)
+
+
+def cg_leg(args, threads):
+    """Config 3: CG on (A^H A + lam I) x = b for the 128x128 L=4 C=4 radial
+    inputs, lam = 1e-3 lambda_max(A^H A), b formed from a smooth x_true by the
+    FP64 NUFFT operator (so x_true is the FP64 solution). FP32 kernel against
+    the BF16 kernel refined by the FP32 operator (restart from its true
+    residual every 10 iterations). Each solve runs the fewest iterations (a
+    multiple of 10, at most 400) whose error against x_true is <= 1e-4; that
+    solve is then timed (HIP-graph-replayed iterations)."""
     import torch
     import mr_aha_amd as mr
     from mr_aha_amd import workloads
     w = workloads.radial_2d(device="cuda")
     s = torch.cuda.Stream()
     cfg = mr.make_config(w["traj_type"], w["n_dim"], w["L"], n_coils=w["n_coils"], stream=s)
-    res = {}
+    out = {}
     with torch.cuda.stream(s):
         K = mr.calc_toep(cfg, w["phi2"], w["traj"])
         op32 = mr.create_op(cfg, w["SEs"], None, None, _phi2_toep=K)
         op16 = mr.create_op(cfg, w["SEs"], None, None, _phi2_toep=K)
         op16.set_kernel_format("bf16")
-        # smooth object: low-pass complex Gaussian coefficients
-        xt = workloads.complex_gaussian(tuple(w["x"].shape), 77, "cuda")
-        f = torch.fft.fftn(xt, dim=(-2, -1))
-        ky = torch.fft.fftfreq(xt.shape[-2], device="cuda").abs()[:, None] < 0.1
-        kx = torch.fft.fftfreq(xt.shape[-1], device="cuda").abs()[None, :] < 0.1
-        xt = torch.fft.ifftn(f * (ky & kx), dim=(-2, -1)).to(torch.complex64).contiguous()
-        Axt = op32(xt)
-        lam = 1e-3 * float(torch.linalg.vector_norm(Axt) / torch.linalg.vector_norm(xt))
-        b = (Axt + lam * xt).contiguous()
-        xref, href = op32.cg(b, lam=lam, iters=200, refresh=10, graph=True)
+        xt = smooth_field(tuple(w["x"].shape), 77, "cuda")
+        lmax = power_lmax(op32, tuple(w["x"].shape), "cuda")
+        lam = 1e-3 * lmax
+        if threads:
+            b = torch.from_numpy(cg_rhs_fp64(w, xt, lam, threads)).cuda()
+            bsrc = "FP64 NUFFT operator (oracle), x_true the FP64 solution"
+        else:
+            b = (op32(xt) + lam * xt).contiguous()
+            bsrc = "FP32 Toeplitz operator (no oracle run), x_true its solution"
+        nx = float(torch.linalg.vector_norm(xt))
         for name, op, ref in (("fp32", op32, None), ("bf16", op16, op32)):
+            its, err = None, None
+            for k in range(10, 401, 10):
+                x, h = op.cg(b, lam=lam, iters=k, refresh=10, op_refresh=ref, graph=True)
+                err = float(torch.linalg.vector_norm(x - xt)) / nx
+                if err <= 1e-4:
+                    its = k
+                    break
+            k = its or 400
             for _ in range(args.warmup):
-                op.cg(b, lam=lam, iters=40, refresh=10, op_refresh=ref, graph=True)
+                op.cg(b, lam=lam, iters=k, refresh=10, op_refresh=ref, graph=True)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(args.steps):
-                x, h = op.cg(b, lam=lam, iters=40, refresh=10, op_refresh=ref, graph=True)
+                x, h = op.cg(b, lam=lam, iters=k, refresh=10, op_refresh=ref, graph=True)
             torch.cuda.synchronize()
             dt = (time.perf_counter() - t0) / args.steps
-            err = float(torch.linalg.vector_norm(x - xref) / torch.linalg.vector_norm(xref))
-            res[name] = {"recon_per_s": round(1.0 / dt, 3), "ms_per_recon": round(dt * 1e3, 3),
-                         "final_residual": h[-1], "error_vs_converged_fp32": err}
+            err = float(torch.linalg.vector_norm(x - xt)) / nx
+            out[name] = {"iterations": k, "reached_1e-4": its is not None,
+                         "error_vs_fp64": err, "final_rel_residual": h[-1],
+                         "recon_per_s": round(1.0 / dt, 2), "ms_per_recon": round(dt * 1e3, 3)}
         op16.close()
         op32.close()
-    line = {"metric": "CG reconstructions/s, 128x128 L4 C4 radial, 40 iterations (FP32 vs bf16 "
-                      "kernel refined against FP32)",
-            "value": res["bf16"]["recon_per_s"], "unit": "reconstructions/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
-            "dtype": "c64 vectors, bf16 kernel, fp64 CG scalars", "data": "synthetic",
-            "config": {"workload": "2D radial 128x128, grid 256x256, L=4, C=4, 64x256 spokes; "
-                                   "CG 40 iterations, refresh 10, lambda 1e-3 |A|"},
-            "fp32": res["fp32"], "bf16": res["bf16"],
-            "speedup_mixed_vs_fp32": round(res["bf16"]["recon_per_s"] / res["fp32"]["recon_per_s"], 3),
-            "reference_residual": href[-1]}
-    print(json.dumps(line), flush=True)
+    out["speedup_mixed_vs_fp32"] = round(out["bf16"]["recon_per_s"] / out["fp32"]["recon_per_s"], 3)
+    out["workload"] = ("2D radial 128x128, grid 256x256, L=4, C=4, 64x256 spokes; "
+                       f"lambda = 1e-3 lambda_max = {lam:.4g}; b from the {bsrc}; "
+                       "refresh every 10 iterations")
+    return out
 
 
-def bench_serve(args):
-    """Poisson mix of CG reconstruction requests (shapes of the serving
-    configuration) through mr_aha_amd.serve, FP32 and then 16-bit kernels."""
+def serve_requests(n_req, rate, seed=2026):
+    """Config 5 request mix: Poisson arrivals; 2D radial {128,192,256,320}^2 or
+    stack {192,256}^2 x {32,48}; L {4,8,16}; C {8,12,16}; 256-512 spokes of
+    2N; 20-50 CG iterations (multiples of 10, so that each solve ends on a
+    true-residual refresh)."""
     import random
-    import torch
-    from mr_aha_amd import serve, workloads
-    rnd = random.Random(2026)
+    rnd = random.Random(seed)
     specs = []
     t = 0.0
-    for i in range(args.requests):
-        t += rnd.expovariate(4.0)               # 4 requests/s on average
+    for i in range(n_req):
+        t += rnd.expovariate(rate)
         if rnd.random() < 0.5:
             n = rnd.choice([128, 192, 256, 320])
             spec = ("2D", (n, n))
         else:
             n = rnd.choice([192, 256])
             spec = ("stack", (n, n, rnd.choice([32, 48])))
-        L = rnd.choice([4, 8, 16])
-        C = rnd.choice([8, 12, 16])
-        ns = rnd.randint(256, 512)
-        specs.append((i, spec, L, C, ns, rnd.randint(20, 50), t))
+        specs.append(dict(i=i, tt=spec[0], nd=spec[1], L=rnd.choice([4, 8, 16]),
+                          C=rnd.choice([8, 12, 16]), ns=rnd.randint(256, 512),
+                          iters=10 * rnd.randint(2, 5), arrival=t))
+    return specs
 
-    def make(prec):
-        reqs = []
-        for i, (tt, nd), L, C, ns, it, arr in specs:
-            n = nd[0]
-            if tt == "2D":
-                w = workloads.radial_2d(n=n, L=L, C=C, n_spokes=ns, nkx=2 * n, n_frames=64,
-                                        seed=900 + i)
-            else:
-                w = workloads.stack_of_stars(n=n, nz=nd[2], L=L, C=C, n_spokes=ns, nkx=2 * n,
-                                             n_frames=64, seed=900 + i)
-            b = workloads.complex_gaussian(tuple(w["x"].shape), 1900 + i, "cuda")
-            reqs.append(serve.Request(rid=i, traj_type=tt, n_dim=nd, L=L, n_coils=C,
-                                      SEs=w["SEs"], phi2=w["phi2"], traj=w["traj"], b=b,
-                                      iters=it, lam=1e-2, refresh=10, precision=prec,
-                                      arrival=arr))
-        return reqs
 
+def serve_leg(args):
+    """Config 5 through mr_aha_amd.serve: every request is a well-posed CG
+    problem, b = (A^H A + lam I) x_true with lam = 0.1 lambda_max (so its
+    20-50 iterations reach 1e-4), formed before the run by the request's own
+    FP32 operator; errors are against x_true. FP32 kernels, then 16-bit
+    kernels refined against FP32 operators."""
+    import torch
+    import mr_aha_amd as mr
+    from mr_aha_amd import serve, workloads
+    specs = serve_requests(args.requests, args.rate)
+    prepared = []
+    t_prep = time.perf_counter()
+    for sp in specs:
+        n = sp["nd"][0]
+        if sp["tt"] == "2D":
+            w = workloads.radial_2d(n=n, L=sp["L"], C=sp["C"], n_spokes=sp["ns"], nkx=2 * n,
+                                    n_frames=64, seed=900 + sp["i"], with_x=False)
+            shape = (sp["L"], n, n)
+        else:
+            w = workloads.stack_of_stars(n=n, nz=sp["nd"][2], L=sp["L"], C=sp["C"],
+                                         n_spokes=sp["ns"], nkx=2 * n, n_frames=64,
+                                         seed=900 + sp["i"], with_x=False)
+            shape = (sp["L"], sp["nd"][2], n, n)
+        xt = smooth_field(shape, 1900 + sp["i"], "cuda")
+        cfg = mr.make_config(w["traj_type"], w["n_dim"], w["L"], n_coils=w["n_coils"],
+                             mem_fraction=0.25)
+        with mr.create_op(cfg, w["SEs"], w["phi2"], w["traj"]) as op:
+            lam = 0.1 * power_lmax(op, shape, "cuda", iters=6, v=xt)
+            b = (op(xt) + lam * xt).contiguous()
+        prepared.append((sp, w, xt, b, lam))
+    torch.cuda.synchronize()
+    t_prep = time.perf_counter() - t_prep
     out = {}
     for prec in ("fp32", "bf16"):
-        reqs = make(prec)
+        reqs = []
+        for sp, w, xt, b, lam in prepared:
+            reqs.append(serve.Request(rid=sp["i"], traj_type=sp["tt"], n_dim=sp["nd"], L=sp["L"],
+                                      n_coils=sp["C"], SEs=w["SEs"], phi2=w["phi2"],
+                                      traj=w["traj"], b=b, iters=sp["iters"], lam=lam,
+                                      refresh=10, precision=prec, arrival=sp["arrival"]))
         torch.cuda.synchronize()
-        done = serve.Server(slots=4).run(reqs, realtime=True)
-        out[prec] = serve.summary(done)
-        out[prec]["errors"] = sorted({r.error for r in done if r.error})[:3]
-        res = sorted(r.hist[-1] for r in done if r.error is None)
-        out[prec]["median_final_residual"] = res[len(res) // 2] if res else None
-        print(f"[serve] {prec}: {out[prec]}", file=sys.stderr, flush=True)
+        done = serve.Server(slots=args.slots).run(reqs, realtime=True)
+        summ = serve.summary(done)
+        errs = []
+        for r, (sp, w, xt, b, lam) in zip(sorted(done, key=lambda r: r.rid), prepared):
+            if r.error is None:
+                errs.append(float(torch.linalg.vector_norm(r.x - xt) /
+                                  torch.linalg.vector_norm(xt)))
+        summ["errors"] = sorted({r.error for r in done if r.error})[:3]
+        summ["max_error_vs_x_true"] = max(errs) if errs else None
+        summ["median_error_vs_x_true"] = sorted(errs)[len(errs) // 2] if errs else None
+        summ["all_within_1e-4"] = bool(errs) and len(errs) == len(done) and max(errs) <= 1e-4
+        out[prec] = summ
+        print(f"[serve] {prec}: {summ}", file=sys.stderr, flush=True)
         del reqs, done
         torch.cuda.empty_cache()
-    line = {"metric": "serving: CG reconstruction requests/s and p50/p99 latency (Poisson mix)",
-            "value": round(out["fp32"]["requests_per_s"], 3), "unit": "requests/s", "n_gpus": 1,
-            "higher_is_better": True, "data": "synthetic",
-            "config": {"workload": f"{args.requests} Poisson requests at 4/s: 2D radial "
-                                   "{128,192,256,320}^2 or stack {192,256}^2 x {32,48}; L {4,8,16}; "
-                                   "C {8,12,16}; 256-512 spokes of 2N; 20-50 CG iterations; 4 slots"},
-            "fp32": out["fp32"], "bf16": out["bf16"]}
-    print(json.dumps(line), flush=True)
+    out["workload"] = (f"{args.requests} Poisson requests at {args.rate}/s, {args.slots} slots: "
+                       "2D radial {128,192,256,320}^2 or stack {192,256}^2 x {32,48}; L {4,8,16}; "
+                       "C {8,12,16}; 256-512 spokes of 2N; 20-50 CG iterations; "
+                       "lam = 0.1 lambda_max; b formed before the run "
+                       f"({t_prep:.0f} s, not timed)")
+    return out
+
+
+def time_applies(op, x, out, steps, warmup):
+    """Warm-up, then `steps` applies timed with HIP events on the device and
+    per-stage events on the library's stream (MR_AHA_HIP_PROFILE)."""
+    import torch
+    from mr_aha_amd import _lib
+    for _ in range(warmup):
+        op.apply(x, out)
+    torch.cuda.synchronize()
+    os.environ["MR_AHA_HIP_PROFILE"] = "1"
+    ms = (ctypes.c_double * 8)()
+    calls = (ctypes.c_int * 8)()
+    _lib.check(_lib.lib.mr_op_profile(op._handle, ms, calls, 8), "mr_op_profile")  # reset
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    t_wall = time.perf_counter()
+    for _ in range(steps):
+        op.apply(x, out)
+    e1.record()
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    os.environ.pop("MR_AHA_HIP_PROFILE")
+    _lib.check(_lib.lib.mr_op_profile(op._handle, ms, calls, 8), "mr_op_profile")
+    return e0.elapsed_time(e1) / steps, t_wall * 1000 / steps, ms, calls
 
 
 def main():
@@ -262,16 +396,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="stack", choices=["stack", "radial", "koosh", "cg", "serve"],
-                    help="stack/radial/koosh: applies/s; cg: CG reconstructions/s, FP32 vs "
-                         "16-bit kernel; serve: request mix, p50/p99 latency")
-    ap.add_argument("--requests", type=int, default=24, help="serve: number of requests")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", default="stack", choices=["stack", "radial", "koosh"],
+                    help="workload of the headline applies/s")
+    ap.add_argument("--requests", type=int, default=200, help="serving leg: number of requests")
+    ap.add_argument("--rate", type=float, default=4.0, help="serving leg: arrivals per second")
+    ap.add_argument("--slots", type=int, default=4, help="serving leg: concurrent requests")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / FP64 oracle")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the 16-bit kernel leg")
+    ap.add_argument("--no-cg", action="store_true", help="skip the CG leg (config 3)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg (config 5)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in tile rules instead of timing the candidates "
                          "(mr_op_autotune) before the warm-up")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16"],
-                    help="kernel storage read by the applies (16-bit: per-4x4-block scales)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -290,10 +426,7 @@ def main():
 
     torch.cuda.set_device(0)
     dev = "cuda:0"
-    if args.config == "cg":
-        return bench_cg(args)
-    if args.config == "serve":
-        return bench_serve(args)
+    threads = 0 if args.no_cpu else host_threads()
     t_gen = time.perf_counter()
     if args.config == "stack":
         w = workloads.stack_of_stars(device=dev)
@@ -318,8 +451,6 @@ def main():
     print(f"[bench] operator created in {t_create:.1f} s (kernel {info['toep_ms'] / 1e3:.1f} s), "
           f"group {info['group'][0]}", file=sys.stderr, flush=True)
 
-    if args.precision != "fp32":
-        op.set_kernel_format(args.precision)
     x = w["x"]
     out = torch.empty_like(x)
     tiles = None
@@ -328,33 +459,13 @@ def main():
         tiles = op.autotune(x, reps=2)
         print(f"[bench] autotuned in {time.perf_counter() - t_tune:.1f} s: {tiles}",
               file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        op.apply(x, out)
-    torch.cuda.synchronize()
-
-    os.environ["MR_AHA_HIP_PROFILE"] = "1"
-    ms = (ctypes.c_double * 8)()
-    calls = (ctypes.c_int * 8)()
-    _lib.check(_lib.lib.mr_op_profile(op._handle, ms, calls, 8), "mr_op_profile")  # reset
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record()
-    t_wall = time.perf_counter()
-    for _ in range(args.steps):
-        op.apply(x, out)
-    e1.record()
-    torch.cuda.synchronize()
-    t_wall = time.perf_counter() - t_wall
-    os.environ.pop("MR_AHA_HIP_PROFILE")
-    _lib.check(_lib.lib.mr_op_profile(op._handle, ms, calls, 8), "mr_op_profile")
-    gpu_ms = e0.elapsed_time(e1)
-    ms_per_step = gpu_ms / args.steps
+    ms_per_step, wall_ms, ms, calls = time_applies(op, x, out, args.steps, args.warmup)
     value = 1000.0 / ms_per_step
 
     group = info["group"][0]
     xbd = int(_lib.lib.mr_op_xbd_group(op._handle))
-    sb = stage_bytes(cfg, group, xbd, args.precision)
+    t_in_use = op.tiles
+    sb = stage_bytes(cfg, group, xbd, "fp32")
     stages = {}
     for i, name in enumerate(_lib.PROFILE_STAGES):
         if calls[i]:
@@ -363,42 +474,88 @@ def main():
             stages[name] = {"avg_ms": round(avg, 4), "launches": int(calls[i]),
                             "gbs": round(nb / (avg * 1e6), 1) if nb else None}
     dom = max((s for s in stages if s in sb), key=lambda s: stages[s]["avg_ms"])
-    dom_kernel = ({"BD": "mr_k_xbd_ct" if xbd else "mr_k_bd"}.get(dom, "mr_fft_axis_fast"))
+    ct_axis = {256, 320, 384, 512, 640, 56}
+    if dom == "BD":
+        dom_kernel = "mr_k_xbd_ct" if xbd else "mr_k_bd"
+    else:
+        dom_kernel = "mr_fft_axis_ct" if set(cfg.phi2_dim[1:]) <= ct_axis else "mr_fft_axis_fast"
     dom_ms = stages[dom]["avg_ms"]
     achieved = sb[dom] / (dom_ms * 1e6)
     B = apply_model_bytes(cfg)
+    print(f"[bench] FP32 kernel: {value:.2f} applies/s, stages {stages}", file=sys.stderr,
+          flush=True)
 
-    cpu = None
-    if not args.no_cpu:
-        K = op.kernel(0)
-        cpu = cpu_baseline(w, K, cfg)
-        del K
+    # mixed precision: the same operator reading the BF16 kernel (per-4x4-block scales)
+    mixed = None
+    if not args.no_mixed:
+        op.set_kernel_format("bf16")
+        mms, _, mmsa, mcalls = time_applies(op, x, out, args.steps, args.warmup)
+        op.set_kernel_format("fp32")
+        mixed = {"kernel": "bf16 entries, fp32 per-4x4-block scales", "applies_per_s":
+                 round(1000.0 / mms, 3), "ms_per_apply": round(mms, 3),
+                 "speedup_vs_fp32": round(ms_per_step / mms, 3),
+                 "fused_pass_ms": round(mmsa[3] / mcalls[3], 3) if mcalls[3] else None}
+        print(f"[bench] BF16 kernel: {mixed}", file=sys.stderr, flush=True)
+
+    # one-coil outputs of the same kernel for the error statements against the
+    # FP64 oracle (coil 0: the CPU leg restricts itself to it)
+    cpu, errs, oracle_info = None, None, None
+    K = op.kernel(0)
     op.close()
+    if threads:
+        cfg1 = mr.make_config(w["traj_type"], w["n_dim"], w["L"], n_coils=1, mem_fraction=0.9)
+        outs = {}
+        with mr.create_op(cfg1, w["SEs"][:1].contiguous(), None, None, _phi2_toep=K) as op1:
+            outs["fp32"] = op1(x).cpu().numpy()
+            if not args.no_mixed:
+                op1.set_kernel_format("bf16")
+                outs["bf16"] = op1(x).cpu().numpy()
+        del K
+        torch.cuda.empty_cache()
+        cpu, errs, oracle_info = cpu_leg(w, outs, threads)
+        print(f"[bench] CPU leg: {cpu}; GPU one-coil errors vs FP64 oracle {errs}",
+              file=sys.stderr, flush=True)
+        if mixed is not None:
+            mixed["error_vs_fp64"] = errs.get("bf16")
+    else:
+        del K
+    del w, x, out
+    torch.cuda.empty_cache()
+
+    cg = None
+    if not args.no_cg:
+        cg = cg_leg(args, threads)
+        print(f"[bench] CG leg: {cg}", file=sys.stderr, flush=True)
+    srv = None
+    if not args.no_serve and args.requests > 0:
+        srv = serve_leg(args)
 
     line = {
         "metric": "A^H A applies/s @320x320x56 L16 C12 (mixed vs FP32); p50/p99 recon latency",
         "value": round(value, 3), "unit": "A^H A applies/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c64 (fp32 complex)" if args.precision == "fp32"
-                 else f"c64 vectors, {args.precision} kernel",
-        "data": "synthetic",
-        "config": {"workload": workload, "precision": f"{args.precision} kernel",
-                   "coil_group": group,
+        "dtype": "c64 (fp32 complex)", "data": "synthetic",
+        "config": {"workload": workload, "precision": "fp32 kernel", "coil_group": group,
                    "pipeline": (f"fused x pass, {xbd} coil(s) per workgroup" if xbd
                                 else "separate FFT and block passes"),
                    "kernel_build_s": round(info["toep_ms"] / 1e3, 2),
-                   "tiles": tiles if tiles is not None else "built-in rules"},
+                   "tiles": t_in_use if tiles is not None else f"built-in rules {t_in_use}"},
+        "error_vs_fp64": errs.get("fp32") if errs else None,
         "roofline": {"bound": "hbm", "kernel": dom_kernel, "stage": dom,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(dom_kernel, workload, args.precision),
+                     "traffic": measured_traffic(dom_kernel, workload, "fp32"),
                      "algorithmic_bytes_per_launch": sb[dom]},
         "apply_model": {"bytes": B, "gbs": round(B / (ms_per_step * 1e6), 1),
                         "frac": round(B / (ms_per_step * 1e6) / HBM_PEAK_GBS, 4)},
         "stages": stages,
-        "wall_ms_per_step": round(t_wall * 1000 / args.steps, 3),
+        "wall_ms_per_step": round(wall_ms, 3),
+        "mixed": mixed,
         "cpu_baseline": cpu,
+        "oracle": oracle_info,
+        "cg": cg,
+        "serve": srv,
     }
     print(json.dumps(line), flush=True)
 

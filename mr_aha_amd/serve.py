@@ -5,9 +5,10 @@ needs an operator (its Toeplitz kernel built on the GPU), a CG solve on the
 normal equations and its memory until it finishes. The server:
 
 * admits pending requests in arrival order while a slot is free and their
-  estimated HBM footprint (kernel, 16-bit copy, plan work buffers, CG
-  vectors, maps) fits the remaining budget, so requests that arrive while
-  others run start as soon as memory and a slot free up;
+  HBM footprint (kernel, 16-bit copy, plan work buffers, the kernel build's
+  gridding chunks, CG vectors, maps; each an upper bound of what the library
+  allocates for them) fits the remaining budget, so requests that arrive
+  while others run start as soon as memory and a slot free up;
 * runs every admitted request on its own non-blocking HIP stream from a
   worker thread (the library calls release the GIL), so kernel builds,
   applies and CG iterations of different requests overlap on the device;
@@ -62,6 +63,33 @@ class Request:
         return self.t_done - self.arrival
 
 
+# Device memory the kernel build of one request may take for its gridding
+# chunks. calc_toep sizes its chunk as at most half of mem_fraction times the
+# free memory, so a build configuration with mem_fraction = 2 BUILD_BYTES /
+# total memory keeps the chunk within BUILD_BYTES whatever is free (or one
+# component, when a single component is larger; build_bytes counts that).
+BUILD_BYTES = 1 << 30
+
+
+def _build_component_bytes(req: Request) -> int:
+    """Upper estimate of one gridded component: the 2x upsampled fine grid
+    (rounded up to whole bins) and the natural-order grid, complex128."""
+    nsp = 2 if req.traj_type == "2D" else 3
+    n = list(req.n_dim) + [1] * (3 - nsp)
+    M = [2 * v for v in n[:nsp]] + [1] * (3 - nsp)
+    if req.traj_type == "stack":
+        M[2] = n[2]
+    tr = 3 if req.traj_type == "3D" else 2
+    fine = 1
+    for a in range(tr):
+        fine *= 2 * M[a] + 64
+    return (fine + M[0] * M[1] * (M[2] if tr == 3 else 1)) * 16
+
+
+def build_bytes(req: Request) -> int:
+    return max(BUILD_BYTES, _build_component_bytes(req))
+
+
 def footprint_bytes(req: Request) -> int:
     """Upper estimate of the request's device memory while it runs."""
     nsp = 2 if req.traj_type == "2D" else 3
@@ -78,9 +106,8 @@ def footprint_bytes(req: Request) -> int:
         nb = ((L + 3) // 4) * ((L + 3) // 4 + 1) // 2
         kern += (T + nb) * P * 4
     work = (n[0] * n[1] * M[2] + n[0] * M[1] * M[2]) * L * C * 8   # fused-pass buffers
-    build = 2 * P * 16 * 4                                          # gridding chunks (FP64)
     vecs = 5 * N * L * 8 + N * C * 8
-    return int(kern + work + build + vecs)
+    return int(kern + work + build_bytes(req) + vecs)
 
 
 class Server:
@@ -108,7 +135,12 @@ class Server:
             with torch.cuda.stream(stream):
                 cfg = api.make_config(req.traj_type, req.n_dim, req.L, n_coils=req.n_coils,
                                       stream=stream, mem_fraction=0.25)
-                K = api.calc_toep(cfg, req.phi2, req.traj)
+                # the kernel build's chunk buffers within build_bytes (see BUILD_BYTES)
+                _, total = torch.cuda.mem_get_info()
+                cfg_build = api.make_config(req.traj_type, req.n_dim, req.L, n_coils=req.n_coils,
+                                            stream=stream,
+                                            mem_fraction=min(0.9, 2.0 * BUILD_BYTES / total))
+                K = api.calc_toep(cfg_build, req.phi2, req.traj)
                 op = api.create_op(cfg, req.SEs, None, None, _phi2_toep=K)
                 op_ref = None
                 try:
