@@ -260,18 +260,6 @@ struct mr_st_z_conj_sum {
 // Block-diagonal multiply
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ long long mr_nat_to_cent_lin(long long loc, const mr_dims &d)
-{
-    const int j0 = (int)(loc % d.m[0]);
-    const long long q = loc / d.m[0];
-    const int j1 = (int)(q % d.m[1]);
-    const int j2 = (int)(q / d.m[1]);
-    int k0 = j0 + d.c_m[0]; if (k0 >= d.m[0]) k0 -= d.m[0];
-    int k1 = j1 + d.c_m[1]; if (k1 >= d.m[1]) k1 -= d.m[1];
-    int k2 = j2 + d.c_m[2]; if (k2 >= d.m[2]) k2 -= d.m[2];
-    return k0 + (long long)d.m[0] * (k1 + (long long)d.m[1] * k2);
-}
-
 // y_c = sum_r v_r A_rc over NC vectors, walking the stored triangle in packed
 // order once: packed element (r, c) adds A_rc v_r to y_c and, off the
 // diagonal, conj(A_rc) v_c to y_r.
@@ -356,50 +344,6 @@ mr_k_bd(float2 *__restrict__ K, const float2 *__restrict__ ker, const int *__res
         mr_bd_walk<L, 1>(v, y, kt + p, TP, d.tri_upper);
 #pragma unroll
         for (int l = 0; l < L; ++l) g[P * l] = y[0][l];
-    }
-}
-
-// Any rank: the rank vector of each location is staged in LDS (blockDim * L
-// values), outputs are formed one column at a time from the packed triangle.
-template <bool MASKED>
-__global__ void __launch_bounds__(64)
-mr_k_bd_any(float2 *__restrict__ K, const float2 *__restrict__ ker, const int *__restrict__ inv,
-            long long kstride, mr_dims d, int ncoil)
-{
-    extern __shared__ __align__(16) unsigned char mr_smem[];
-    const int L = d.L;
-    float2 *v = reinterpret_cast<float2 *>(mr_smem) + (size_t)threadIdx.x * L;
-    const long long loc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (loc >= d.pad_size) return;
-    const long long P = d.pad_size;
-    const float2 *base;
-    if (MASKED) {
-        const int m = inv[loc];
-        if (m < 0) {
-            for (int c = 0; c < ncoil; ++c)
-                for (int l = 0; l < L; ++l) K[loc + P * (l + (long long)L * c)] = make_float2(0.f, 0.f);
-            return;
-        }
-        base = ker + m;
-    } else {
-        base = ker + mr_nat_to_cent_lin(loc, d);
-    }
-    for (int c = 0; c < ncoil; ++c) {
-        float2 *g = K + loc + P * (long long)L * c;
-        for (int l = 0; l < L; ++l) v[l] = g[P * l];
-        for (int i = 0; i < L; ++i) {
-            float2 acc = make_float2(0.f, 0.f);
-            for (int j = 0; j < L; ++j) {
-                // element (j, i) of the block
-                int r = j, cc = i, cj = 0;
-                if (d.tri_upper) { if (r > cc) { int t = r; r = cc; cc = t; cj = 1; } }
-                else { if (r < cc) { int t = r; r = cc; cc = t; cj = 1; } }
-                const int t = d.tri_upper ? cc * (cc + 1) / 2 + r : cc * L - cc * (cc - 1) / 2 + (r - cc);
-                const float2 a = base[kstride * t];
-                acc = cj ? cmacc(acc, a, v[j]) : cmac(acc, a, v[j]);
-            }
-            g[P * i] = acc;
-        }
     }
 }
 
@@ -620,7 +564,7 @@ struct mr_aha_plan_s {
     int tile[4];          // pencils per workgroup of Fo, Fy, Iy, Io (0: default rule)
     int no_wide;          // 1: no two-coil 1024-thread fused workgroups
     mr_xbd_ct_fn xbd_mf;  // xbd_ct with the block step on the matrix cores, or null
-    int no_mf;            // 1: keep the FMA block walks (autotuner's choice)
+    int no_mf;            // 1: keep the FMA block walks (built-in rule; the autotuner may pick xbd_mf)
     bool tuning;          // mr_aha_plan_autotune is timing stages
     int kf;               // kernel format used by execute (MR_KF_*)
     const unsigned *kq;   // 16-bit entries and block scales (kf != FP32)
@@ -790,11 +734,13 @@ static int mr_xbd_setup(mr_aha_plan p)
     }
     // The same kernel with its block step on the matrix cores: FP32 kernel,
     // unmasked, L a multiple of 4 and an even kx centering shift (adjacent
-    // locations then have adjacent kernel entries). MR_AHA_HIP_MFMA=0 keeps
-    // the FMA walks.
+    // locations then have adjacent kernel entries). It runs when
+    // MR_AHA_HIP_MFMA=1 was set at plan creation or the autotuner measured it
+    // faster; the built-in rule keeps the FMA walks (measured faster at
+    // 320x320x56, L=16: the matrix-core step loads every off-diagonal entry
+    // twice, once per output it feeds).
     mr_xbd_ct_fn mf = nullptr;
-    if (ct && !cfg.is_masked && p->kf == MR_KF_FP32 && L % 4 == 0 && (Mx / 2) % 2 == 0 &&
-        !p->no_mf && mr_env_ll("MR_AHA_HIP_MFMA", 1) != 0) {
+    if (ct && !cfg.is_masked && p->kf == MR_KF_FP32 && L % 4 == 0 && (Mx / 2) % 2 == 0) {
         int mgmax = 0;
         mf = (nt > MR_XBD_THREADS) ? mr_xbd_ct_pick_wide(Mx, L, false, p->kf, true)
                                    : mr_xbd_ct_pick(Mx, L, false, p->kf, &mgmax, true);
@@ -821,6 +767,8 @@ static int mr_plan_create_impl(const mr_config *cfg, mr_aha_plan *out)
     if (!p) MR_FAIL(MR_ERR_ALLOC, "failed to allocate the plan struct");
     p->cfg = *cfg;
     p->stream = (hipStream_t)cfg->stream;
+    // matrix-core block step of the fused pass: opt-in (or the autotuner's pick)
+    p->no_mf = mr_env_ll("MR_AHA_HIP_MFMA", 0) == 1 ? 0 : 1;
     mr_dims_from_config(cfg, &p->dims);
     int rc = mr_fft_get<float>(cfg->phi2_dim[0], &p->fx);
     if (rc == MR_SUCCESS) rc = mr_fft_get<float>(cfg->phi2_dim[1], &p->fy);
@@ -931,6 +879,9 @@ extern "C" MR_EXPORT int mr_aha_plan_destroy(mr_aha_plan plan)
     return MR_SUCCESS;
 }
 
+int mr_bdt_launch(const mr_dims &d, float2 *K, const float2 *ker, const int *inv,
+                  long long kstride, int nc, bool masked, hipStream_t s);
+
 static int mr_blockdiag(mr_aha_plan p, float2 *K, int nc, const float2 *ker)
 {
     const mr_dims d = p->dims;
@@ -958,19 +909,8 @@ static int mr_blockdiag(mr_aha_plan p, float2 *K, int nc, const float2 *ker)
         MR_LAUNCH_CHECK("mr_k_bd");
         return MR_SUCCESS;
     }
-    const size_t lds = (size_t)64 * d.L * sizeof(float2);
-    if (lds > 64 * 1024)
-        MR_FAIL(MR_ERR_UNSUPPORTED, "L = %d needs %zu bytes of LDS per 64 locations", d.L, lds);
-    const long long blocks = mr_ceil_div(d.pad_size, 64);
-    MR_GRID_CHECK(blocks, 64, "mr_k_bd_any");
-    if (masked)
-        hipLaunchKernelGGL(mr_k_bd_any<true>, dim3((unsigned)blocks), dim3(64), lds, p->stream, K,
-                           ker, p->d_inv, kstride, d, nc);
-    else
-        hipLaunchKernelGGL(mr_k_bd_any<false>, dim3((unsigned)blocks), dim3(64), lds, p->stream, K,
-                           ker, p->d_inv, kstride, d, nc);
-    MR_LAUNCH_CHECK("mr_k_bd_any");
-    return MR_SUCCESS;
+    // ranks without a compiled kernel: the tiled kernel (LDS panels)
+    return mr_bdt_launch(d, K, ker, p->d_inv, kstride, nc, masked, p->stream);
 }
 
 // x-last pipeline for one coil group (coils coil0 .. coil0 + nc - 1):
@@ -1030,8 +970,9 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
         if (p->xbd_ct) {
             // the matrix-core block step reads kernel entries two locations
             // per 16-byte load
-            const mr_xbd_ct_fn fn =
-                (p->xbd_mf && ((unsigned long long)ker & 15) == 0) ? p->xbd_mf : p->xbd_ct;
+            const mr_xbd_ct_fn fn = (p->xbd_mf && !p->no_mf && ((unsigned long long)ker & 15) == 0)
+                                        ? p->xbd_mf
+                                        : p->xbd_ct;
             hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3((unsigned)p->xbd_nt), p->xbd_lds,
                                s, B, ker, p->d_inv, kstride, d, p->fx.tws, R, G, nc, ncg, p->kq,
                                p->ksc);
@@ -1287,15 +1228,12 @@ static int mr_autotune_impl(mr_aha_plan p, const mr_complex *x, const mr_complex
     // block step on the matrix cores against the FMA walks
     if (rc == MR_SUCCESS && p->xbd_mf) {
         double mf_ms = 0.0, fma_ms = 0.0;
+        p->no_mf = 0;
         rc = stage_ms(MR_ST_BD, &mf_ms);
         if (rc == MR_SUCCESS) {
             p->no_mf = 1;
-            rc = mr_xbd_setup(p);
-            if (rc == MR_SUCCESS) rc = stage_ms(MR_ST_BD, &fma_ms);
-            if (rc == MR_SUCCESS && fma_ms >= mf_ms) {
-                p->no_mf = 0;
-                rc = mr_xbd_setup(p);
-            }
+            rc = stage_ms(MR_ST_BD, &fma_ms);
+            if (rc == MR_SUCCESS && mf_ms < fma_ms) p->no_mf = 0;
         }
     }
     p->tuning = false;
@@ -1331,7 +1269,8 @@ extern "C" MR_EXPORT int mr_aha_plan_tiles(mr_aha_plan plan, int *tiles, int n)
     MR_REQUIRE(plan != nullptr, "mr_aha_plan_tiles: plan is NULL");
     MR_REQUIRE(tiles != nullptr || n == 0, "mr_aha_plan_tiles: tiles is NULL");
     const int v[6] = {plan->tile[0], plan->tile[1], plan->tile[2], plan->tile[3],
-                      plan->xbd_nt > MR_XBD_THREADS ? 1 : 0, plan->xbd_mf ? 1 : 0};
+                      plan->xbd_nt > MR_XBD_THREADS ? 1 : 0,
+                      (plan->xbd_mf && !plan->no_mf) ? 1 : 0};
     for (int i = 0; i < n && i < 6; ++i) tiles[i] = v[i];
     return MR_SUCCESS;
 }

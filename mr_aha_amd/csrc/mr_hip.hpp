@@ -198,3 +198,16 @@ template <typename V, bool FAST> struct mr_st_crop {
     }
     __device__ void flush(long long, int, V *) const {}
 };
+
+// Kernel (centered) linear index of natural-order grid location loc.
+__device__ __forceinline__ long long mr_nat_to_cent_lin(long long loc, const mr_dims &d)
+{
+    const int j0 = (int)(loc % d.m[0]);
+    const long long q = loc / d.m[0];
+    const int j1 = (int)(q % d.m[1]);
+    const int j2 = (int)(q / d.m[1]);
+    int k0 = j0 + d.c_m[0]; if (k0 >= d.m[0]) k0 -= d.m[0];
+    int k1 = j1 + d.c_m[1]; if (k1 >= d.m[1]) k1 -= d.m[1];
+    int k2 = j2 + d.c_m[2]; if (k2 >= d.m[2]) k2 -= d.m[2];
+    return k0 + (long long)d.m[0] * (k1 + (long long)d.m[1] * k2);
+}
