@@ -95,16 +95,41 @@ def apply_model_bytes(cfg):
 
 
 def calibrate_copy(nbytes=1 << 30, reps=4):
-    """Plain device copy of a known size, the reference point for scaling the
-    rocprofv3 FETCH_SIZE / WRITE_SIZE readings of the library's kernels."""
+    """Device copies of a known size.
+
+    * torch copy_ of `nbytes` `reps` times: the reference point for scaling
+      the rocprofv3 FETCH_SIZE / WRITE_SIZE readings of the library's kernels;
+    * the library's streaming copy (mr_stream_copy: 16-byte non-temporal
+      loads and stores) of 8 GiB, timed with HIP events: the attainable HBM
+      bandwidth (bytes read + written per second)."""
     import torch
+    from mr_aha_amd import _lib
     x = torch.empty(nbytes // 8, dtype=torch.complex64, device="cuda")
     y = torch.empty_like(x)
     for _ in range(reps):
         y.copy_(x)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": reps}),
-          flush=True)
+    del x, y
+    big = 8 << 30
+    a = torch.empty(big // 8, dtype=torch.complex64, device="cuda")
+    b = torch.empty_like(a)
+    s = torch.cuda.current_stream()
+    for _ in range(2):
+        _lib.check(_lib.lib.mr_stream_copy(a.data_ptr(), b.data_ptr(), big, s.cuda_stream),
+                   "mr_stream_copy")
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    n = 10
+    e0.record()
+    for _ in range(n):
+        _lib.check(_lib.lib.mr_stream_copy(a.data_ptr(), b.data_ptr(), big, s.cuda_stream),
+                   "mr_stream_copy")
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / n
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": reps,
+                      "stream_copy_bytes": big, "stream_copy_ms": round(ms, 3),
+                      "stream_copy_gbs": round(2 * big / (ms * 1e6), 1)}), flush=True)
 
 
 def measured_traffic(kernel, workload, precision="fp32"):

@@ -113,6 +113,12 @@ MR_EXPORT int mr_op_cg_noalloc(mr_op op, mr_op op_refresh, const mr_complex *d_b
                                mr_complex *d_x, double lambda, int n_iter, int refresh,
                                int flags, double *res_hist);
 
+/* Streaming copy of `bytes` (a multiple of 16, 16-byte aligned buffers) on
+ * `stream`: 16-byte non-temporal loads and stores, the attainable HBM
+ * bandwidth that the passes' achieved bandwidth is compared with (benchmark
+ * calibration). No counterpart in the reference. */
+MR_EXPORT int mr_stream_copy(const void *d_src, void *d_dst, int64_t bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -198,6 +198,7 @@ _decl("mr_op_cg", ctypes.c_int, [_p, _p, _p, _p, ctypes.c_double, ctypes.c_int, 
                                  ctypes.c_int, ctypes.POINTER(ctypes.c_double)])
 _decl("mr_op_cg_noalloc", ctypes.c_int, [_p, _p, _p, _p, ctypes.c_double, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)])
+_decl("mr_stream_copy", ctypes.c_int, [_p, _p, ctypes.c_int64, _p])
 CG_ZERO_INIT = 1
 CG_GRAPH = 2
 _decl("mr_op_set_kernel_format", ctypes.c_int, [_p, ctypes.c_int])

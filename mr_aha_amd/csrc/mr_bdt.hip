@@ -7,7 +7,7 @@
 // coil of the group. Layout here: K [P, L, ncoil] (natural order), kernel
 // [T, kstride] centered (or [T, num_masked] with the inverse mask map).
 //
-// A workgroup of 256 work-items (four wavefronts) owns a tile of TP = 16
+// A workgroup of 512 work-items (eight wavefronts) owns a tile of TP = 16
 // consecutive locations and a batch of NCB coils:
 //  * the batch's inputs v[b][r][p] are staged in LDS once (location fastest:
 //    the global reads and the LDS reads are both unit-stride over p);
@@ -15,7 +15,7 @@
 //    contiguous range of packed offsets); a panel is copied once from HBM
 //    into LDS for the TP locations, tile[e][p], 16 consecutive kernel entries
 //    (128 bytes) per packed offset;
-//  * work-item (p, g) owns location p and outputs o = g, g + 16, ...; from
+//  * work-item (p, g) owns location p and outputs o = g, g + 32, ...; from
 //    each panel it adds the column o (y_o += A_ro v_r, rows r of column o)
 //    and the mirrors of the panel's other columns (y_o += conj(A_oc) v_c);
 //    every output touches exactly L entries, so the work-items stay
@@ -23,9 +23,11 @@
 //    and from LDS by the two work-items whose outputs it feeds;
 //  * each entry read from LDS serves the NCB coils of the batch from
 //    registers.
-// OPT (outputs per work-item, a power of two >= ceil(L/16)) and NCB are
+// OPT (outputs per work-item, a power of two >= ceil(L/32)) and NCB are
 // template parameters so that the accumulators stay in registers:
-// NCB * OPT <= 32.
+// NCB * OPT <= 16. The panel capacity keeps a workgroup within half of the
+// LDS where the rank allows, so that two workgroups (16 wavefronts) share a
+// CU and hide each other's LDS and HBM latency.
 
 #include "mr_hip.hpp"
 
@@ -33,7 +35,7 @@
 
 namespace {
 
-constexpr int BDT_THREADS = 256;
+constexpr int BDT_THREADS = 512;
 constexpr int BDT_TP = 16;                      // locations per tile
 constexpr int BDT_NG = BDT_THREADS / BDT_TP;    // work-items per location
 
@@ -104,6 +106,7 @@ mr_k_bdt(float2 *__restrict__ K, const float2 *__restrict__ ker, const int *__re
                     const int r0 = upper ? 0 : o;
                     const int len = bdt_collen(o, L, upper);
                     const float2 *e = pt + (size_t)(bdt_colstart(o, L, upper) - t0) * BDT_TP + p;
+#pragma unroll 2
                     for (int i = 0; i < len; ++i) {
                         const float2 a = e[(size_t)i * BDT_TP];
 #pragma unroll
@@ -145,10 +148,10 @@ typedef void (*mr_bdt_fn)(float2 *, const float2 *, const int *, long long, mr_d
 
 template <int OPT, bool MASKED> mr_bdt_fn bdt_pick_ncb(int ncb)
 {
-    if constexpr (OPT <= 8) {
+    if constexpr (OPT <= 4) {
         if (ncb >= 4) return mr_k_bdt<OPT, 4, MASKED>;
     }
-    if constexpr (OPT <= 16) {
+    if constexpr (OPT <= 8) {
         if (ncb >= 2) return mr_k_bdt<OPT, 2, MASKED>;
     }
     return mr_k_bdt<OPT, 1, MASKED>;
@@ -157,11 +160,11 @@ template <int OPT, bool MASKED> mr_bdt_fn bdt_pick_ncb(int ncb)
 template <bool MASKED> mr_bdt_fn bdt_pick(int opt, int ncb)
 {
     switch (opt) {
+    case 1: return bdt_pick_ncb<1, MASKED>(ncb);
     case 2: return bdt_pick_ncb<2, MASKED>(ncb);
     case 4: return bdt_pick_ncb<4, MASKED>(ncb);
     case 8: return bdt_pick_ncb<8, MASKED>(ncb);
-    case 16: return bdt_pick_ncb<16, MASKED>(ncb);
-    default: return bdt_pick_ncb<32, MASKED>(ncb);
+    default: return bdt_pick_ncb<16, MASKED>(ncb);
     }
 }
 
@@ -174,16 +177,14 @@ int mr_bdt_launch(const mr_dims &d, float2 *K, const float2 *ker, const int *inv
 {
     const int L = d.L;
     MR_REQUIRE(L >= 1 && L <= 512, "block multiply: rank L = %d outside 1..512", L);
-    int opt = 2;
+    int opt = 1;
     while (opt * BDT_NG < L) opt *= 2;
-    int ncb = 32 / opt;
+    int ncb = 16 / opt;
     if (ncb > 4) ncb = 4;
     if (ncb > nc) ncb = nc;
     const long long cap = mr_env_ll("MR_AHA_HIP_BDT_NCB", 4);
     if (ncb > cap) ncb = (int)(cap < 1 ? 1 : cap);
     ncb = ncb >= 4 ? 4 : (ncb >= 2 ? 2 : 1);
-    if (opt > 8 && ncb > 2) ncb = 2;
-    if (opt > 16) ncb = 1;
     int dev = 0, maxlds = 0;
     MR_HIP_CHECK(hipGetDevice(&dev));
     MR_HIP_CHECK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
@@ -191,8 +192,12 @@ int mr_bdt_launch(const mr_dims &d, float2 *K, const float2 *ker, const int *inv
     const size_t room = (size_t)maxlds - 256;   // static kb[]
     MR_REQUIRE(vbytes + (size_t)L * BDT_TP * sizeof(float2) <= room,
                "block multiply: L = %d does not fit the %d bytes of LDS", L, maxlds);
-    // panel capacity: the rest of the LDS, at least one whole column
-    int E = (int)((room - vbytes) / (BDT_TP * sizeof(float2)));
+    // panel capacity: within half of the LDS (two workgroups per CU) when a
+    // whole column still fits, else the rest of it
+    const size_t half = (size_t)maxlds / 2 - 256;
+    const size_t pbytes =
+        (vbytes + (size_t)L * BDT_TP * sizeof(float2) <= half ? half : room) - vbytes;
+    int E = (int)(pbytes / (BDT_TP * sizeof(float2)));
     const int T = L * (L + 1) / 2;
     if (E > T) E = T;
     const size_t lds = vbytes + (size_t)E * BDT_TP * sizeof(float2);

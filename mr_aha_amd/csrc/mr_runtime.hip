@@ -95,3 +95,52 @@ void mr_fft_release_all()
     for (auto &kv : g_tw) (void)hipFree(kv.second);
     g_tw.clear();
 }
+
+// ---------------------------------------------------------------------------
+// Streaming copy: the attainable HBM bandwidth that the passes are measured
+// against (read + write of `bytes`). 16-byte vector loads and stores with the
+// non-temporal hint, eight per work-item in flight, a grid of a few
+// workgroups per CU striding over the buffer.
+// ---------------------------------------------------------------------------
+namespace {
+typedef unsigned int mr_cp_u4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) mr_k_stream_copy(const mr_cp_u4 *__restrict__ src,
+                                                         mr_cp_u4 *__restrict__ dst, long long n)
+{
+    constexpr int U = 8;
+    const long long stride = (long long)gridDim.x * 256 * U;
+    for (long long b = (long long)blockIdx.x * 256 * U + threadIdx.x; b < n; b += stride) {
+        mr_cp_u4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (b + (long long)u * 256 < n) v[u] = __builtin_nontemporal_load(src + b + (long long)u * 256);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (b + (long long)u * 256 < n) __builtin_nontemporal_store(v[u], dst + b + (long long)u * 256);
+    }
+}
+}  // namespace
+
+extern "C" MR_EXPORT int mr_stream_copy(const void *d_src, void *d_dst, int64_t bytes,
+                                        void *stream)
+{
+    mr_clear_error();
+    MR_REQUIRE(d_src != nullptr && d_dst != nullptr, "mr_stream_copy: NULL buffer");
+    MR_REQUIRE(bytes >= 0 && bytes % 16 == 0, "mr_stream_copy: %lld bytes is not a multiple of 16",
+               (long long)bytes);
+    MR_REQUIRE(((uintptr_t)d_src & 15) == 0 && ((uintptr_t)d_dst & 15) == 0,
+               "mr_stream_copy: buffers must be 16-byte aligned");
+    if (bytes == 0) return MR_SUCCESS;
+    int dev = 0, ncu = 0;
+    MR_HIP_CHECK(hipGetDevice(&dev));
+    MR_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const long long n = bytes / 16;
+    long long blocks = (n + 256 * 8 - 1) / (256 * 8);
+    const long long cap = (long long)ncu * 8;
+    if (blocks > cap) blocks = cap;
+    hipLaunchKernelGGL(mr_k_stream_copy, dim3((unsigned)blocks), dim3(256), 0,
+                       (hipStream_t)stream, (const mr_cp_u4 *)d_src, (mr_cp_u4 *)d_dst, n);
+    MR_LAUNCH_CHECK("mr_k_stream_copy");
+    return MR_SUCCESS;
+}
