@@ -28,6 +28,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# read + write bandwidth of the library's streaming copy on one MI355X
+# (python bench.py --calibrate-copy; profiles/stream_copy_ceiling.md)
+COPY_CEILING_GBS = 5240.3
 
 
 def stage_bytes(cfg, nc, xbd, precision="fp32"):
@@ -132,18 +135,23 @@ def calibrate_copy(nbytes=1 << 30, reps=4):
                       "stream_copy_gbs": round(2 * big / (ms * 1e6), 1)}), flush=True)
 
 
-def measured_traffic(kernel, workload, precision="fp32"):
-    """HBM bytes per launch of `kernel` on `workload` with the kernel stored
-    in `precision`, from the committed rocprofv3 record
-    (profiles/hbm_traffic.json), or None."""
+def profile_record(kernel, workload, precision="fp32"):
+    """The committed rocprofv3 record of `kernel` on `workload` with the
+    kernel stored in `precision` (profiles/hbm_traffic.json), or {}."""
     path = ROOT / "profiles" / "hbm_traffic.json"
     if not path.exists():
-        return None
+        return {}
     for rec in json.loads(path.read_text()).get("records", []):
         if (rec.get("kernel") == kernel and rec.get("workload") == workload
                 and rec.get("precision", "fp32") == precision):
-            return int(rec["hbm_bytes_per_launch"])
-    return None
+            return rec
+    return {}
+
+
+def measured_traffic(kernel, workload, precision="fp32"):
+    """HBM bytes per launch from the committed record, or None."""
+    rec = profile_record(kernel, workload, precision)
+    return int(rec["hbm_bytes_per_launch"]) if "hbm_bytes_per_launch" in rec else None
 
 
 def host_threads() -> int:
@@ -421,8 +429,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="stack", choices=["stack", "radial", "koosh"],
-                    help="workload of the headline applies/s")
+    ap.add_argument("--config", default="stack", choices=["stack", "radial", "koosh", "2d72"],
+                    help="workload of the applies/s (stack: the headline; 2d72: the reference's "
+                         "large-rank shape, random kernel; the other legs run with stack only)")
+    ap.add_argument("--masked", action="store_true",
+                    help="2d72: the masked operator (mask of the 754 x 320 radial trajectory)")
     ap.add_argument("--requests", type=int, default=200, help="serving leg: number of requests")
     ap.add_argument("--rate", type=float, default=4.0, help="serving leg: arrivals per second")
     ap.add_argument("--slots", type=int, default=4, help="serving leg: concurrent requests")
@@ -451,9 +462,25 @@ def main():
 
     torch.cuda.set_device(0)
     dev = "cuda:0"
+    if args.config != "stack":
+        # the mixed, CPU/oracle, CG and serving legs belong to the headline run
+        args.no_mixed = args.no_cpu = args.no_cg = args.no_serve = True
     threads = 0 if args.no_cpu else host_threads()
     t_gen = time.perf_counter()
-    if args.config == "stack":
+    K_given, mask = None, None
+    if args.config == "2d72":
+        # src/tests/bench_large_rank.py's shape: 320^2 image, 640^2 grid, L=72,
+        # 12 coils, 754 golden-angle spokes x 320 samples (for the mask); the
+        # apply's time does not depend on the kernel's values, so it is random
+        n, L, C = 320, 72, 12
+        g = torch.Generator(device=dev).manual_seed(72)
+        traj = workloads.radial_traj(754, n, dev)
+        w = dict(traj_type="2D", n_dim=(n, n), L=L, n_coils=C, traj=traj, phi2=None,
+                 SEs=workloads.complex_gaussian((C, n, n), 2, dev),
+                 x=workloads.complex_gaussian((L, n, n), 1, dev))
+        workload = ("2d72: 2D 320x320, grid 640x640, L=72, C=12, random kernel" +
+                    (", masked (754x320 radial spokes)" if args.masked else ""))
+    elif args.config == "stack":
         w = workloads.stack_of_stars(device=dev)
         workload = "stack-of-stars 320x320x56, grid 640x640x56, L=16, C=12, 400x640 spokes"
     elif args.config == "radial":
@@ -469,7 +496,20 @@ def main():
     print(f"[bench] {workload}: inputs generated in {t_gen:.1f} s", file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
-    op = mr.create_op(cfg, w["SEs"], w["phi2"], w["traj"])
+    if args.config == "2d72":
+        if args.masked:
+            cfg = mr.make_config("2D", w["n_dim"], w["L"], n_coils=w["n_coils"], is_masked=True,
+                                 mem_fraction=0.9)
+            mask = mr.calc_mask(cfg, w["traj"])
+            cfg = mr.make_config("2D", w["n_dim"], w["L"], n_coils=w["n_coils"], is_masked=True,
+                                 num_masked=int(mask.numel()), mem_fraction=0.9)
+        T = w["L"] * (w["L"] + 1) // 2
+        cols = mask.numel() if mask is not None else 4 * w["n_dim"][0] * w["n_dim"][1]
+        K_given = torch.complex(torch.randn((T, cols), generator=g, device=dev),
+                                torch.randn((T, cols), generator=g, device=dev))
+        op = mr.create_op(cfg, w["SEs"], None, None, mask, _phi2_toep=K_given)
+    else:
+        op = mr.create_op(cfg, w["SEs"], w["phi2"], w["traj"])
     torch.cuda.synchronize()
     t_create = time.perf_counter() - t0
     info = op.info
@@ -501,7 +541,7 @@ def main():
     dom = max((s for s in stages if s in sb), key=lambda s: stages[s]["avg_ms"])
     ct_axis = {256, 320, 384, 512, 640, 56}
     if dom == "BD":
-        dom_kernel = "mr_k_xbd_ct" if xbd else "mr_k_bd"
+        dom_kernel = "mr_k_xbd_ct" if xbd else ("mr_k_bd" if cfg.L <= 16 else "mr_k_bdt")
     else:
         dom_kernel = "mr_fft_axis_ct" if set(cfg.phi2_dim[1:]) <= ct_axis else "mr_fft_axis_fast"
     dom_ms = stages[dom]["avg_ms"]
@@ -571,7 +611,11 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(dom_kernel, workload, "fp32"),
-                     "algorithmic_bytes_per_launch": sb[dom]},
+                     "algorithmic_bytes_per_launch": sb[dom],
+                     # the byte model's fraction; the kernel's measured limiter
+                     # (rocprofv3 counters, committed record) when there is one
+                     "limiter": profile_record(dom_kernel, workload).get("limiter"),
+                     "attainable_copy_gbs": COPY_CEILING_GBS},
         "apply_model": {"bytes": B, "gbs": round(B / (ms_per_step * 1e6), 1),
                         "frac": round(B / (ms_per_step * 1e6) / HBM_PEAK_GBS, 4)},
         "stages": stages,
