@@ -468,6 +468,10 @@ def main():
         args.no_cg = args.no_serve = True
         if args.config == "2d72":
             args.no_mixed = args.no_cpu = True
+        if args.config == "koosh":
+            # the 3D CPU NUFFT of 6.4M samples (and its FP64 oracle at w = 10)
+            # takes minutes per coil: no CPU leg
+            args.no_cpu = True
     threads = 0 if args.no_cpu else host_threads()
     t_gen = time.perf_counter()
     K_given, mask = None, None
