@@ -56,6 +56,10 @@ class Request:
     hist: List[float] = field(default_factory=list)
     t_admit: float = 0.0
     t_done: float = 0.0
+    # device time of the request's first and last work on its stream (HIP
+    # events), seconds after the run's reference event
+    d_start: float = 0.0
+    d_end: float = 0.0
     error: Optional[str] = None
 
     @property
@@ -126,11 +130,14 @@ class Server:
         self._used = 0
         self._running = 0
 
-    def _solve(self, req: Request, t0: float) -> None:
+    def _solve(self, req: Request, t0: float, ref_ev) -> None:
         # a non-blocking stream: the graph capture of one request must not be
         # joined by legacy-stream work that another request's thread issues
         handle = _hiprt.stream_create_nonblocking()
         stream = torch.cuda.ExternalStream(handle)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
         try:
             with torch.cuda.stream(stream):
                 cfg = api.make_config(req.traj_type, req.n_dim, req.L, n_coils=req.n_coils,
@@ -160,7 +167,10 @@ class Server:
         except Exception as e:  # reported per request
             req.error = f"{type(e).__name__}: {e}"
         finally:
+            ev1.record(stream)
             stream.synchronize()
+            req.d_start = ref_ev.elapsed_time(ev0) / 1e3
+            req.d_end = ref_ev.elapsed_time(ev1) / 1e3
             _hiprt.stream_destroy(handle)
         req.t_done = time.perf_counter() - t0
 
@@ -173,6 +183,8 @@ class Server:
             if footprint_bytes(r) > self.budget:
                 r.error = f"footprint {footprint_bytes(r)} B exceeds the budget {self.budget} B"
         pending = [r for r in order if r.error is None]
+        ref_ev = torch.cuda.Event(enable_timing=True)
+        ref_ev.record()
         t0 = time.perf_counter()
         done = threading.Semaphore(0)
 
@@ -201,7 +213,7 @@ class Server:
                 head.t_admit = time.perf_counter() - t0
                 if not realtime:
                     head.arrival = 0.0
-                fut = pool.submit(self._solve, head, t0)
+                fut = pool.submit(self._solve, head, t0, ref_ev)
                 fut.add_done_callback(lambda f, r=head: finish(r))
                 admitted += 1
             for _ in range(admitted):
@@ -219,6 +231,23 @@ def percentile(vals, q):
     return v[lo] + (v[hi] - v[lo]) * (k - lo)
 
 
+def device_overlap(reqs: List[Request]) -> float:
+    """Fraction of the device-busy time (the union of the requests' device
+    intervals) during which two or more requests had work on the device."""
+    ev = sorted([(r.d_start, 1) for r in reqs] + [(r.d_end, -1) for r in reqs])
+    busy = multi = 0.0
+    active = 0
+    last = ev[0][0] if ev else 0.0
+    for t, d in ev:
+        if active >= 1:
+            busy += t - last
+        if active >= 2:
+            multi += t - last
+        active += d
+        last = t
+    return multi / busy if busy > 0 else 0.0
+
+
 def summary(reqs: List[Request]) -> dict:
     ok = [r for r in reqs if r.error is None]
     lat = [r.latency for r in ok]
@@ -226,4 +255,5 @@ def summary(reqs: List[Request]) -> dict:
     return dict(requests=len(reqs), completed=len(ok), failed=len(reqs) - len(ok),
                 requests_per_s=len(ok) / span if ok and span > 0 else float("nan"),
                 p50_s=percentile(lat, 50), p99_s=percentile(lat, 99),
-                mean_s=sum(lat) / len(lat) if lat else float("nan"))
+                mean_s=sum(lat) / len(lat) if lat else float("nan"),
+                device_overlap=device_overlap(ok))

@@ -1,5 +1,5 @@
 """Ranks above the compiled block kernels (L = 17 .. 512): the tiled block
-multiply (panels of stored columns staged in LDS, coil batches of 1, 2 or 4).
+multiply (panels of stored columns staged in LDS, coil batches of up to 16).
 
 Ranks of the reference's own rank checks (tests/check_bd_kernels.py:52: 17,
 20, 30-33, 47, 72, 130) plus 256 and 512, both triangles, masked and
@@ -38,7 +38,10 @@ def host(t):
 # the full L x L block of every location)
 RANKS = [(17, (12, 10), 3), (20, (12, 10), 5), (30, (10, 8), 3), (31, (10, 8), 2),
          (32, (10, 8), 4), (33, (10, 8), 3), (47, (8, 8), 5), (72, (8, 6), 3),
-         (130, (6, 5), 2), (256, (4, 4), 2), (512, (4, 3), 1)]
+         (130, (6, 5), 2), (256, (4, 4), 2), (512, (4, 3), 1),
+         # one batch of 12 coils (16-coil instantiation), and 20 coils in two
+         # batches of 10
+         (40, (8, 6), 12), (20, (10, 8), 20)]
 
 
 @pytest.mark.parametrize("L,n,C", RANKS)
@@ -68,7 +71,7 @@ def test_large_rank_block_multiply(mr, monkeypatch, L, n, C, masked):
                              num_masked=int(mask.size) if masked else 0)
         ref = toeplitz.apply(g, x, S, K.astype(np.complex128), upper=(tri == "upper"), mask=mask)
         outs = []
-        for ncb in ("4", "2", "1"):
+        for ncb in ("16", "4", "2", "1"):
             monkeypatch.setenv("MR_AHA_HIP_BDT_NCB", ncb)
             y = host(mr.aha_toeplitz(cfg, dx, dS, dK, dmask))
             outs.append(y)
