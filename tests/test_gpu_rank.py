@@ -1,11 +1,13 @@
 """Ranks above the compiled block kernels (L = 17 .. 512): the tiled block
-multiply (panels of stored columns staged in LDS, coil batches of up to 16).
+multiply (outputs x coils blocked in registers, kernel entries streamed,
+coil batches of up to 12).
 
 Ranks of the reference's own rank checks (tests/check_bd_kernels.py:52: 17,
 20, 30-33, 47, 72, 130) plus 256 and 512, both triangles, masked and
 unmasked, random packed kernels with complex diagonals, against the FP64
-five-stage apply (relative 2-norm 1e-5); every coil-batch size against the
-others (1e-6: the same sums in the same order, so in practice bit-equal).
+five-stage apply (relative 2-norm 1e-5); every coil-batch size and output
+blocking against the others (1e-6: the same sums in the same order, so in
+practice bit-equal).
 """
 
 import numpy as np
@@ -39,8 +41,7 @@ def host(t):
 RANKS = [(17, (12, 10), 3), (20, (12, 10), 5), (30, (10, 8), 3), (31, (10, 8), 2),
          (32, (10, 8), 4), (33, (10, 8), 3), (47, (8, 8), 5), (72, (8, 6), 3),
          (130, (6, 5), 2), (256, (4, 4), 2), (512, (4, 3), 1),
-         # one batch of 12 coils (16-coil instantiation), and 20 coils in two
-         # batches of 10
+         # one batch of 12 coils, and 20 coils in batches
          (40, (8, 6), 12), (20, (10, 8), 20)]
 
 
@@ -71,8 +72,9 @@ def test_large_rank_block_multiply(mr, monkeypatch, L, n, C, masked):
                              num_masked=int(mask.size) if masked else 0)
         ref = toeplitz.apply(g, x, S, K.astype(np.complex128), upper=(tri == "upper"), mask=mask)
         outs = []
-        for ncb in ("16", "4", "2", "1"):
+        for ncb, opt in (("12", "0"), ("4", "0"), ("2", "0"), ("1", "0"), ("12", "4")):
             monkeypatch.setenv("MR_AHA_HIP_BDT_NCB", ncb)
+            monkeypatch.setenv("MR_AHA_HIP_BDT_OPT", opt)
             y = host(mr.aha_toeplitz(cfg, dx, dS, dK, dmask))
             outs.append(y)
         e = rel(outs[0], ref)
