@@ -1050,26 +1050,26 @@ static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, cons
             mr_zmap zm{V, lgTXZ};
             mr_ld_sx_z ld{x, S, zm, NxNy, N, Nz, Nz, d.c_n[2], L, coil0};
             mr_st_z st{A, zm, NxNy, Nz};
-            MR_TRY((mr_fft_pass<float, false>(p->fz, ld, st, NxNy * V, 0, 1 << lgTXZ, s, "Z1")));
+            MR_TRY((mr_fft_pass_ct<false>(p->fz, ld, st, NxNy * V, 0, 1 << lgTXZ, s, "Z1")));
             MR_TRY(mr_prof_mark(p, MR_ST_F3));
             // F1: zero-padded x-FFT -> B [Mx, Ny, Nz, V]
             mr_ld_pad<float2, true> l1{A, 1, Mx, Nx, d.c_n[0]};
             mr_st_axis<float2, true> s1{B, 1, Mx};
-            MR_TRY((mr_fft_pass<float, false>(p->fx, l1, s1, NyNz * V, 0, 8, s, "F1")));
+            MR_TRY((mr_fft_pass_ct<false>(p->fx, l1, s1, NyNz * V, 0, 8, s, "F1")));
             MR_TRY(mr_prof_mark(p, MR_ST_F1));
             std::swap(A, B);  // x-transformed rows now in A
         } else {
             // F1: pad * S, x-FFT -> A [Mx, Ny, Nz, V]
             mr_ld_pad_sx ld{x, S, N, NyNz, V, Nx, Mx, d.c_n[0], L, coil0};
             mr_st_rows_vfast st{A, NyNz, V, Mx};
-            MR_TRY((mr_fft_pass<float, false>(p->fx, ld, st, NyNz * V, 0, 8, s, "F1")));
+            MR_TRY((mr_fft_pass_ct<false>(p->fx, ld, st, NyNz * V, 0, 8, s, "F1")));
             MR_TRY(mr_prof_mark(p, MR_ST_F1));
         }
         // F2: y-FFT -> B [Mx, My, Nz, V]
         {
             mr_ld_pad<float2, false> ld{A, Mx, My, Ny, d.c_n[1]};
             mr_st_axis<float2, false> st{B, Mx, My};
-            MR_TRY((mr_fft_pass<float, false>(p->fy, ld, st, (long long)Mx * Nz * V, 0, 16, s, "F2")));
+            MR_TRY((mr_fft_pass_ct<false>(p->fy, ld, st, (long long)Mx * Nz * V, 0, 16, s, "F2")));
             MR_TRY(mr_prof_mark(p, MR_ST_F2));
         }
         float2 *K = B, *free_buf = A;
@@ -1077,7 +1077,7 @@ static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, cons
             // F3: z-FFT -> A [Mx, My, Mz, V]
             mr_ld_pad<float2, false> ld{B, (long long)Mx * My, Mz, Nz, d.c_n[2]};
             mr_st_axis<float2, false> st{A, (long long)Mx * My, Mz};
-            MR_TRY((mr_fft_pass<float, false>(p->fz, ld, st, (long long)Mx * My * V, 0, 64, s, "F3")));
+            MR_TRY((mr_fft_pass_ct<false>(p->fz, ld, st, (long long)Mx * My * V, 0, 64, s, "F3")));
             MR_TRY(mr_prof_mark(p, MR_ST_F3));
             K = A;
             free_buf = B;
@@ -1089,7 +1089,7 @@ static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, cons
             // I3: inverse z-FFT, keep Nz planes -> free_buf [Mx, My, Nz, V]
             mr_ld_axis<float2, false> ld{K, (long long)Mx * My, Mz};
             mr_st_crop<float2, false> st{free_buf, (long long)Mx * My, Mz, Nz, d.c_n[2]};
-            MR_TRY((mr_fft_pass<float, true>(p->fz, ld, st, (long long)Mx * My * V, 0, 64, s, "I3")));
+            MR_TRY((mr_fft_pass_ct<true>(p->fz, ld, st, (long long)Mx * My * V, 0, 64, s, "I3")));
             MR_TRY(mr_prof_mark(p, MR_ST_I3));
             src = free_buf;
             free_buf = K;
@@ -1099,26 +1099,26 @@ static int mr_execute_impl(mr_aha_plan p, const float2 *x, const float2 *S, cons
         {
             mr_ld_axis<float2, false> ld{src, Mx, My};
             mr_st_crop<float2, false> st{dst, Mx, My, Ny, d.c_n[1]};
-            MR_TRY((mr_fft_pass<float, true>(p->fy, ld, st, (long long)Mx * Nz * V, 0, 16, s, "I2")));
+            MR_TRY((mr_fft_pass_ct<true>(p->fy, ld, st, (long long)Mx * Nz * V, 0, 16, s, "I2")));
             MR_TRY(mr_prof_mark(p, MR_ST_I2));
         }
         if (zfirst) {
             // I1: inverse x-FFT, crop -> src [Nx, Ny, Nz, V]
             mr_ld_axis<float2, true> l1{dst, 1, Mx};
             mr_st_crop<float2, true> s1{src, 1, Mx, Nx, d.c_n[0]};
-            MR_TRY((mr_fft_pass<float, true>(p->fx, l1, s1, NyNz * V, 0, 8, s, "I1")));
+            MR_TRY((mr_fft_pass_ct<true>(p->fx, l1, s1, NyNz * V, 0, 8, s, "I1")));
             MR_TRY(mr_prof_mark(p, MR_ST_I1));
             // Z2: inverse z-FFT, conj(S_c), coil sum -> out
             mr_zmap zm{L, lgTXZ};
             mr_ld_z_coil ld{src, zm, NxNy, Nz, L, nc};
             mr_st_z_conj_sum st{out, S, zm, NxNy, N, Nz, Nz, d.c_n[2], coil0, coil0 > 0 ? 1 : 0};
-            MR_TRY((mr_fft_pass<float, true>(p->fz, ld, st, NxNy * L, (size_t)Nz, 1 << lgTXZ, s, "Z2")));
+            MR_TRY((mr_fft_pass_ct<true>(p->fz, ld, st, NxNy * L, (size_t)Nz, 1 << lgTXZ, s, "Z2")));
             MR_TRY(mr_prof_mark(p, MR_ST_I3));
         } else {
             // I1: inverse x-FFT, crop, conj(S) and coil sum -> out
             mr_ld_row_coil ld{dst, NyNz, Mx, L, nc};
             mr_st_crop_conj_sum st{out, S, N, NyNz, Nx, Mx, d.c_n[0], coil0, coil0 > 0 ? 1 : 0, L};
-            MR_TRY((mr_fft_pass<float, true>(p->fx, ld, st, NyNz * L, (size_t)Nx, 8, s, "I1")));
+            MR_TRY((mr_fft_pass_ct<true>(p->fx, ld, st, NyNz * L, (size_t)Nx, 8, s, "I1")));
             MR_TRY(mr_prof_mark(p, MR_ST_I1));
         }
     }
