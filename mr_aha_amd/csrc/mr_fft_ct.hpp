@@ -326,6 +326,245 @@ mr_fft_axis_ct(const float2 *tws, Load ld, Store st, long long npencil, int lgTX
     st.flush(p0, tp, acc);
 }
 
+// ---------------------------------------------------------------------------
+// Double-buffered 16-pencil axis pass with direct global-to-LDS loads.
+//
+// For passes whose 16 pencils of a tile are consecutive along the contiguous
+// axis (row stride I a multiple of 16), every source row of a tile is one
+// 128-byte line. A workgroup owns one CU (two 16·N·8-byte buffers, 160 KB at
+// N = 640) and walks tiles t, t + grid, ...: while it transforms and stores
+// tile t from one buffer, the rows of tile t + grid stream into the other
+// with global_load_lds_dwordx4 (16 bytes per lane straight into LDS at
+// base + 16·lane, no VGPRs), eight rows per wavefront instruction. Buffers
+// hold [element][16 pencils], so a wavefront's reads and writes in the
+// stages are 128-byte rows of 16 pencils (no bank conflicts, no swizzle).
+// The butterflies, twiddles and their products are those of mr_ct_stage,
+// so results are bit-identical to the one-tile kernels.
+
+// Source rows of a tile and the elements they fill: row s (0 <= s < rows)
+// of pencil p0 lands at element (s - c) mod N.
+template <class Load> struct mr_dma_rows { static constexpr bool kOk = false; };
+template <> struct mr_dma_rows<mr_ld_pad<float2, false>> {
+    static constexpr bool kOk = true;
+    using L = mr_ld_pad<float2, false>;
+    __host__ __device__ static int rows(const L &l) { return l.Ns; }
+    __host__ __device__ static int shift(const L &l) { return l.c; }
+    __host__ __device__ static long long stride(const L &l) { return l.I; }
+    __host__ __device__ static const float2 *base(const L &l) { return l.in; }
+    __device__ static const float2 *row0(const L &l, long long p0) { return l.in + l.pencil(p0, 0); }
+};
+template <> struct mr_dma_rows<mr_ld_axis<float2, false>> {
+    static constexpr bool kOk = true;
+    using L = mr_ld_axis<float2, false>;
+    __host__ __device__ static int rows(const L &l) { return l.n; }
+    __host__ __device__ static int shift(const L &) { return 0; }
+    __host__ __device__ static long long stride(const L &l) { return l.I; }
+    __host__ __device__ static const float2 *base(const L &l) { return l.in; }
+    __device__ static const float2 *row0(const L &l, long long p0) { return l.in + l.pencil(p0, 0); }
+};
+template <class Store> struct mr_dma_store { static constexpr bool kOk = false; };
+template <> struct mr_dma_store<mr_st_axis<float2, false>> { static constexpr bool kOk = true; };
+template <> struct mr_dma_store<mr_st_crop<float2, false>> { static constexpr bool kOk = true; };
+
+// Workgroup barrier over LDS traffic only: this wavefront's LDS operations
+// complete, then s_barrier. __syncthreads() would also drain the row loads
+// still streaming into the other buffer (its release fence covers every LDS
+// write, and a global-to-LDS load is one), so the overlap would be lost. No
+// memory access moves across it (memory clobber); the row loads are awaited
+// explicitly where their buffer is read.
+__device__ __forceinline__ void mr_lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Butterflies per work-item and stage, and stages, at most (lengths 512 and
+// 640 with 1024 work-items).
+constexpr int MR_EP_KMAX = 2;
+constexpr int MR_EP_NST = 4;
+
+// The twiddle of each of a work-item's butterflies, for every stage: the
+// same for every tile, so they are loaded once per workgroup and no vector
+// memory load (which completes in order) waits behind the next tile's rows.
+template <int N, int S, int NT, class TW>
+__device__ __forceinline__ void mr_ep_twload(float2 (&w)[MR_EP_NST][MR_EP_KMAX], const TW &tw)
+{
+    constexpr mr_ct_fac F = mr_ct_factor(N);
+    static_assert(F.rest == 1 && F.nst <= MR_EP_NST, "length must be 2^a 3^b 5^c 7^d");
+    if constexpr (S < F.nst) {
+        constexpr int R = F.r[S], NS = F.ns[S], m = N / R, NB = 16 * m;
+        constexpr int KMAX = (NB + NT - 1) / NT;
+        static_assert(KMAX <= MR_EP_KMAX, "butterflies per work-item");
+        static_assert(mr_cdiv_exact<NS>(), "inexact constant division");
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+                const int f = (int)threadIdx.x + k * NT;
+                if (f < NB) {
+                    const unsigned j = (unsigned)(f >> 4);
+                    const unsigned kk = j - mr_cdiv<NS>::q(j) * NS;
+                    w[S][k] = tw.get(NS + kk - 1, 0u);
+                }
+            }
+        }
+        mr_ep_twload<N, S + 1, NT>(w, tw);
+    }
+}
+
+// One Stockham stage over [element][16 pencils]: work-item f takes pencil
+// f & 15 and butterfly f >> 4, so 16 neighbouring lanes touch one row.
+// OOP: read from src, write to A (no barrier between the two phases).
+template <int N, int S, bool INV, int NT, bool OOP = false>
+__device__ __forceinline__ void mr_ep_stage(float2 *A, const float2 (&ws)[MR_EP_KMAX],
+                                            const float2 *src = nullptr)
+{
+    constexpr mr_ct_fac F = mr_ct_factor(N);
+    constexpr int R = F.r[S], NS = F.ns[S];
+    constexpr int m = N / R;
+    constexpr int NB = 16 * m;
+    constexpr int KMAX = (NB + NT - 1) / NT;
+    constexpr int RW = (R > 1) ? R - 1 : 1;
+    float2 v[KMAX][R];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        const int f = (int)threadIdx.x + k * NT;
+        if (f < NB) {
+            const int p = f & 15, j = f >> 4;
+            const float2 *in = OOP ? src : A;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[k][r] = in[(j + r * m) * 16 + p];
+        }
+    }
+    if constexpr (!OOP) mr_lds_barrier();
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        const int f = (int)threadIdx.x + k * NT;
+        if (f < NB) {
+            const int p = f & 15, j = f >> 4;
+            const unsigned jn = mr_cdiv<NS>::q((unsigned)j), kk = (unsigned)j - jn * NS;
+            if constexpr (NS > 1) {
+                float2 wp[RW];
+                wp[0] = ws[k];
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    if (r >= 2) wp[r - 1] = cmul(wp[r / 2 - 1], wp[r - r / 2 - 1]);
+                    v[k][r] = INV ? cmulc(wp[r - 1], v[k][r]) : cmul(wp[r - 1], v[k][r]);
+                }
+            }
+            bfly<R, INV>(v[k]);
+#pragma unroll
+            for (int q = 0; q < R; ++q) A[(jn * (NS * R) + kk + q * NS) * 16 + p] = v[k][q];
+        }
+    }
+    mr_lds_barrier();
+}
+
+// Stages S .. last, in place.
+template <int N, int S, bool INV, int NT>
+__device__ __forceinline__ void mr_ep_fft(float2 *A, const float2 (&w)[MR_EP_NST][MR_EP_KMAX])
+{
+    constexpr mr_ct_fac F = mr_ct_factor(N);
+    if constexpr (S < F.nst) {
+        mr_ep_stage<N, S, INV, NT>(A, w[S]);
+        mr_ep_fft<N, S + 1, INV, NT>(A, w);
+    }
+}
+
+template <int N, bool INV, class Load, class Store>
+__global__ void __launch_bounds__(mr_w16_threads<N>())
+mr_fft_axis_dma(const float2 *tws, Load ld, Store st, long long ntiles)
+{
+    constexpr int NT = mr_w16_threads<N>();
+    constexpr int NW = NT / 64;
+    using RW = mr_dma_rows<Load>;
+    // rows land in B; the first stage reads B and writes A, the others work
+    // in place on A. Two distinct LDS objects, so that the compiler sees that
+    // the next tile's row loads into B cannot alias the stages' accesses to A
+    // (and does not wait for them there).
+    __shared__ __align__(16) float2 A[16 * N];
+    __shared__ __align__(16) float2 B[16 * N];
+    float2 w[MR_EP_NST][MR_EP_KMAX];
+    mr_ep_twload<N, 0, NT>(w, mr_tw_buf{mr_rsrc(tws, (unsigned)N * 8u)});
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ns = RW::rows(ld), c = RW::shift(ld);
+    const long long rs = RW::stride(ld);
+    // rows [0, na) land at elements N - c + s, rows [na, ns) at s - c; both
+    // runs are whole groups of 8 rows (checked by the host)
+    const int na = c < ns ? c : ns;
+    const int ngrp = ns >> 3;
+    long long t = blockIdx.x;
+    if (t >= ntiles) return;
+    // elements that no source row fills (zero padding): N - ns of them from
+    // (ns - c) mod N on; the row loads never write them, so once is enough
+    {
+        int z0 = ns - c;
+        if (z0 < 0) z0 += N;
+        if (z0 >= N) z0 -= N;
+        for (int i = threadIdx.x; i < (N - ns) * 16; i += NT) {
+            int e = z0 + (i >> 4);
+            if (e >= N) e -= N;
+            B[e * 16 + (i & 15)] = make_float2(0.f, 0.f);
+        }
+    }
+    auto fill = [&](long long tt) {
+        const float2 *src = RW::row0(ld, tt * 16) + (lane & 7) * 2;
+        for (int gi = wave; gi < ngrp; gi += NW) {
+            const int s0 = gi * 8;
+            const int e0 = s0 < na ? s0 - c + N : s0 - c;
+            __builtin_amdgcn_global_load_lds(
+                src + rs * (s0 + (lane >> 3)),
+                (__attribute__((address_space(3))) void *)(B + e0 * 16), 16, 0, 0);
+        }
+    };
+    fill(t);
+    const int p = threadIdx.x & 15;
+    for (; t < ntiles; t += gridDim.x) {
+        __builtin_amdgcn_s_waitcnt(0);   // this wavefront's row loads into B
+        __syncthreads();                 // everyone's (and the zero rows)
+        mr_ep_stage<N, 0, INV, NT, true>(A, w[0], B);   // B -> A, ends in a barrier
+        if (t + gridDim.x < ntiles) fill(t + gridDim.x);
+        mr_ep_fft<N, 1, INV, NT>(A, w);
+        const typename Store::PB sb = st.pencil(t * 16 + p, 0);
+        for (int e = threadIdx.x >> 4; e < N; e += NT / 16)
+            st.put(sb, p, e, 0, A[e * 16 + p], nullptr);
+        mr_lds_barrier();                // A is read out before the next first stage
+    }
+}
+
+// Launch the double-buffered pass when the tile rows are 128-byte lines
+// (returns false to use the one-tile kernels).
+template <int N, bool INV, class Load, class Store>
+bool mr_fft_pass_dma(const mr_fft_len<float> &f, const Load &ld, const Store &st,
+                     long long npencil, hipStream_t stream)
+{
+    if constexpr (!mr_dma_rows<Load>::kOk || !mr_dma_store<Store>::kOk || N < 512) {
+        return false;
+    } else {
+        using RW = mr_dma_rows<Load>;
+        constexpr size_t lds = (size_t)2 * 16 * N * sizeof(float2);
+        if (lds > 160 * 1024 || mr_env_ll("MR_AHA_HIP_FFT_DMA", 1) == 0) return false;
+        const int ns = RW::rows(ld), c = RW::shift(ld);
+        if (npencil % 16 != 0 || RW::stride(ld) % 16 != 0 || ns % 8 != 0 || c % 8 != 0 ||
+            c < 0 || c >= N || ns > N || ((unsigned long long)RW::base(ld) & 15) != 0)
+            return false;
+        static int ncu = 0;
+        if (ncu == 0) {
+            int dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
+                    hipSuccess || ncu <= 0)
+                ncu = 256;
+        }
+        // workgroups per CU that fit the LDS (two at N <= 320)
+        const long long ntiles = npencil / 16;
+        const long long per_cu = (long long)(160 * 1024 / lds);
+        const long long grid = ntiles < ncu * per_cu ? ntiles : ncu * per_cu;
+        hipLaunchKernelGGL((mr_fft_axis_dma<N, INV, Load, Store>), dim3((unsigned)grid),
+                           dim3(mr_w16_threads<N>()), 0, stream, f.tws, ld, st, ntiles);
+        return true;
+    }
+}
+
 // Host: launch the compile-time kernel for length N (tile choice as in
 // mr_fft_pass).
 template <int N, bool INV, class Load, class Store>
@@ -356,6 +595,10 @@ int mr_fft_pass_ct_n(const mr_fft_len<float> &f, const Load &ld, const Store &st
                                 (int)need16) == hipSuccess;
         if (w16_ok && TX >= 16 && acc_per_pencil == 0 && npencil >= 16 &&
             mr_env_ll("MR_AHA_HIP_FFT_W16", 1) != 0) {
+        if (mr_fft_pass_dma<N, INV>(f, ld, st, npencil, stream)) {
+            MR_LAUNCH_CHECK(name);
+            return MR_SUCCESS;
+        }
         const long long blocks = (npencil + 15) / 16;
         const long long maxb16 = mr_max_blocks(mr_w16_threads<N>());
         for (long long b0 = 0; b0 < blocks; b0 += maxb16) {
