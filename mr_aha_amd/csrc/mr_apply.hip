@@ -913,6 +913,221 @@ static int mr_blockdiag(mr_aha_plan p, float2 *K, int nc, const float2 *ker)
     return mr_bdt_launch(d, K, ker, p->d_inv, kstride, nc, masked, p->stream);
 }
 
+// ---------------------------------------------------------------------------
+// Double-buffered outer (z) passes of the x-last route (stack of stars: a
+// compiled z length with Nz = Mz), built like mr_fft_axis_dma. A tile is 16
+// consecutive xy positions x 8 ranks = 128 pencils of [element][pencil]; its
+// source rows (16 xy, one 128-byte line each) stream straight into LDS with
+// global_load_lds_dwordx4 while the previous tile is transformed and
+// stored. The first radix stage reads the loaded rows and writes the work
+// buffer, so the next tile's rows can land as soon as it is done.
+// ---------------------------------------------------------------------------
+
+constexpr int MR_ZD_LB = 8;              // ranks per tile
+constexpr int MR_ZD_PT = 16 * MR_ZD_LB;  // pencils per tile
+// LDS stride of one rank's N rows of 16 xy, padded by one row so that the
+// four ranks a wavefront reads at once start on alternate bank halves
+template <int N> constexpr int mr_zd_xs() { return N * 16 + 16; }
+
+// Forward: element e of pencil (rank l, xy) is S_c x_l at image z = (e + cnz)
+// mod N (the centering of mr_ld_sx_z). Per tile (xy tile q, rank block lb)
+// the coils in turn: the tile's x rows are loaded once and serve every coil;
+// coil c's S_c rows stream in while coil c-1 is transformed and stored (two
+// LDS buffers), and the next tile's x rows while the last coil is. Output
+// [plane, N, V] (v = l + L c) as mr_st_z.
+template <int N>
+__global__ void __launch_bounds__(1024)
+mr_k_fo_dma(const float2 *tws, const float2 *__restrict__ x, const float2 *__restrict__ S,
+            float2 *__restrict__ out, long long NxNy, long long Nimg, int L, int nc, int coil0,
+            int cnz, long long ntiles)
+{
+    constexpr int NT = 1024, NW = NT / 64, PT = MR_ZD_PT, XS = mr_zd_xs<N>(), NG = N / 8;
+    static_assert(N % 8 == 0, "rows load in groups of 8");
+    __shared__ __align__(16) float2 Bx[MR_ZD_LB * XS];   // [rank][z][xy]
+    __shared__ __align__(16) float2 Bs0[N * 16];         // [z][xy], even coils
+    __shared__ __align__(16) float2 Bs1[N * 16];         // odd coils
+    __shared__ __align__(16) float2 A[N * PT];           // [e][rank * 16 + xy]
+    float2 w[MR_EP_NST][MR_EP_KMAX];
+    mr_ep_twload<N, 0, NT, PT>(w, mr_tw_buf{mr_rsrc(tws, (unsigned)N * 8u)});
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nlb = L / MR_ZD_LB;
+    const int col = (lane & 7) * 2, row = lane >> 3;
+    // the x rows of tile t (8 ranks x N rows), and/or the S_c rows
+    auto fill = [&](long long t, int c, float2 *bs, bool with_x) {
+        const int lb = (int)(t % nlb);
+        const long long xy0 = (t / nlb) * 16 + col;
+        const int g0 = with_x ? 0 : MR_ZD_LB * NG;
+        for (int gi = g0 + wave; gi < (MR_ZD_LB + 1) * NG; gi += NW) {
+            const int rk = gi / NG, z0 = (gi - rk * NG) * 8;   // rk == 8: the S_c rows
+            if (rk < MR_ZD_LB)
+                __builtin_amdgcn_global_load_lds(
+                    x + xy0 + NxNy * (z0 + row) + Nimg * (lb * MR_ZD_LB + rk),
+                    (__attribute__((address_space(3))) void *)(Bx + rk * XS + z0 * 16), 16, 0, 0);
+            else
+                __builtin_amdgcn_global_load_lds(
+                    S + xy0 + NxNy * (z0 + row) + Nimg * (coil0 + c),
+                    (__attribute__((address_space(3))) void *)(bs + z0 * 16), 16, 0, 0);
+        }
+    };
+    const int p = threadIdx.x & (PT - 1);
+    auto coil = [&](long long t, int c, const float2 *bcur, float2 *bnext) {
+        __builtin_amdgcn_s_waitcnt(0);   // this wavefront's row loads
+        __syncthreads();                 // everyone's
+        mr_ep_stage<N, 0, false, NT, PT, true>(A, w[0], [&](int pp, int e) {
+            int z = e + cnz;
+            if (z >= N) z -= N;
+            const int xy = pp & 15;
+            return cmul(bcur[z * 16 + xy], Bx[(pp >> 4) * XS + z * 16 + xy]);
+        });
+        if (c + 1 < nc) fill(t, c + 1, bnext, false);
+        else if (t + gridDim.x < ntiles) fill(t + gridDim.x, 0, bnext, true);
+        mr_ep_fft<N, 1, false, NT, PT>(A, w);
+        const long long v = (t % nlb) * MR_ZD_LB + (p >> 4) + (long long)L * c;
+        float2 *o = out + (t / nlb) * 16 + (p & 15) + NxNy * (long long)N * v;
+        for (int e = threadIdx.x / PT; e < N; e += NT / PT) o[NxNy * e] = A[e * PT + p];
+        mr_lds_barrier();                // A and bcur are read out before they are refilled
+    };
+    long long t = blockIdx.x;
+    if (t >= ntiles) return;
+    fill(t, 0, Bs0, true);
+    bool odd = false;   // the current coil's S rows are in Bs1
+    for (; t < ntiles; t += gridDim.x) {
+        for (int c = 0; c < nc; ++c) {
+            if (!odd) coil(t, c, Bs0, Bs1);
+            else coil(t, c, Bs1, Bs0);
+            odd = !odd;
+        }
+    }
+}
+
+// Inverse: per tile (xy tile q, rank block lb) the coils in turn; coil c's
+// rows [plane, N, V] (v = l + L c) and its S_c rows stream in while coil c-1
+// is transformed. The output element e is image z = (e + cnz) mod N; each
+// work-item keeps the coil sum of conj(S_c) times its elements in registers
+// (the order of mr_st_z_conj_sum) and writes AhAx[xy, z, l] (added to it when
+// `accumulate`) after the last coil. The S rows of consecutive coils
+// alternate between two LDS buffers.
+template <int N>
+__global__ void __launch_bounds__(1024)
+mr_k_io_dma(const float2 *tws, const float2 *__restrict__ in, const float2 *__restrict__ S,
+            float2 *__restrict__ out, long long NxNy, long long Nimg, int L, int nc, int coil0,
+            int cnz, int accumulate, long long ntiles)
+{
+    constexpr int NT = 1024, NW = NT / 64, PT = MR_ZD_PT, XS = mr_zd_xs<N>(), NG = N / 8;
+    constexpr int KE = N * PT / NT;   // elements per work-item
+    static_assert(N % 8 == 0 && (N * PT) % NT == 0, "tile shape");
+    __shared__ __align__(16) float2 Bx[MR_ZD_LB * XS];   // [rank][e][xy]
+    __shared__ __align__(16) float2 Bs0[N * 16];         // [z][xy], even coils
+    __shared__ __align__(16) float2 Bs1[N * 16];         // odd coils
+    __shared__ __align__(16) float2 W[N * PT];           // [e][rank * 16 + xy]
+    float2 w[MR_EP_NST][MR_EP_KMAX];
+    mr_ep_twload<N, 0, NT, PT>(w, mr_tw_buf{mr_rsrc(tws, (unsigned)N * 8u)});
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nlb = L / MR_ZD_LB;
+    const int p = threadIdx.x & (PT - 1), xy = p & 15, rl = p >> 4;
+    const int e0 = threadIdx.x >> 7;   // elements e0 + 8k, k < KE
+    auto fill = [&](long long t, int c, float2 *bs) {
+        const int lb = (int)(t % nlb);
+        const long long xy0 = (t / nlb) * 16 + (lane & 7) * 2;
+        const int row = lane >> 3;
+        for (int gi = wave; gi < (MR_ZD_LB + 1) * NG; gi += NW) {
+            const int rk = gi / NG, z0 = (gi - rk * NG) * 8;   // rk == 8: the S_c rows
+            if (rk < MR_ZD_LB)
+                __builtin_amdgcn_global_load_lds(
+                    in + xy0 + NxNy * (z0 + row) +
+                        NxNy * (long long)N * (lb * MR_ZD_LB + rk + (long long)L * c),
+                    (__attribute__((address_space(3))) void *)(Bx + rk * XS + z0 * 16), 16, 0, 0);
+            else
+                __builtin_amdgcn_global_load_lds(
+                    S + xy0 + NxNy * (z0 + row) + Nimg * (coil0 + c),
+                    (__attribute__((address_space(3))) void *)(bs + z0 * 16), 16, 0, 0);
+        }
+    };
+    // one coil of tile t: first stage from the loaded rows, then the next
+    // (tile, coil)'s rows start loading, then the other stages and the coil
+    // sum with this coil's S rows
+    float2 acc[KE];
+    auto coil = [&](long long t, int c, const float2 *bcur, float2 *bnext) {
+        __builtin_amdgcn_s_waitcnt(0);   // this wavefront's row loads
+        __syncthreads();                 // everyone's
+        mr_ep_stage<N, 0, true, NT, PT, true>(W, w[0], [&](int pp, int e) {
+            return Bx[(pp >> 4) * XS + e * 16 + (pp & 15)];
+        });
+        long long tn = t;
+        int cn = c + 1;
+        if (cn == nc) {
+            cn = 0;
+            tn = t + gridDim.x;
+        }
+        if (tn < ntiles) fill(tn, cn, bnext);
+        mr_ep_fft<N, 1, true, NT, PT>(W, w);
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            int z = e0 + 8 * k + cnz;
+            if (z >= N) z -= N;
+            const float2 v = cmulc(bcur[z * 16 + xy], W[(e0 + 8 * k) * PT + p]);
+            acc[k] = (c == 0) ? v : cadd(acc[k], v);
+        }
+        mr_lds_barrier();                // W and bcur are read out before they are refilled
+    };
+    long long t = blockIdx.x;
+    if (t >= ntiles) return;
+    fill(t, 0, Bs0);
+    bool odd = false;   // the current coil's S rows are in Bs1
+    for (; t < ntiles; t += gridDim.x) {
+        for (int c = 0; c < nc; ++c) {
+            if (!odd) coil(t, c, Bs0, Bs1);
+            else coil(t, c, Bs1, Bs0);
+            odd = !odd;
+        }
+        const long long q = t / nlb;
+        const int l = (int)(t % nlb) * MR_ZD_LB + rl;
+        float2 *o = out + q * 16 + xy + Nimg * l;
+        if (accumulate) {   // a later coil group: add to the earlier groups' sum
+#pragma unroll
+            for (int k = 0; k < KE; ++k) {
+                int z = e0 + 8 * k + cnz;
+                if (z >= N) z -= N;
+                o[NxNy * z] = cadd(o[NxNy * z], acc[k]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < KE; ++k) {
+                int z = e0 + 8 * k + cnz;
+                if (z >= N) z -= N;
+                o[NxNy * z] = acc[k];
+            }
+        }
+    }
+}
+
+// Run Fo / Io on the double-buffered kernels when the shape allows (z length
+// 56 = Mz = Nz, whole 16-position xy tiles, ranks in blocks of 8, 16-byte
+// aligned arrays); false: use the one-tile passes.
+static bool mr_zd_ok(const mr_dims &d, int L, long long plane, const void *a, const void *b,
+                     const void *c)
+{
+    return d.m[2] == 56 && d.n[2] == 56 && plane % 16 == 0 && L % MR_ZD_LB == 0 &&
+           d.image_size % 2 == 0 && (((unsigned long long)a | (unsigned long long)b |
+                                      (unsigned long long)c) & 15) == 0 &&
+           mr_env_ll("MR_AHA_HIP_ZDMA", 1) != 0;
+}
+
+static long long mr_zd_grid(long long ntiles)
+{
+    static int ncu = 0;
+    if (ncu == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
+                hipSuccess || ncu <= 0)
+            ncu = 256;
+    }
+    return ntiles < ncu ? ntiles : ncu;
+}
+
 // x-last pipeline for one coil group (coils coil0 .. coil0 + nc - 1):
 //   Fo   outer axis (z; y in 2D): S_c x_l, zero-pad, FFT      -> A [Nx, Ny, Mz, V]
 //   Fy   y-FFT, zero-padding the Ny rows (3D and stack)      -> B [Nx, My, Mz, V]
@@ -946,7 +1161,14 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
         // MR_AHA_HIP_FO_TX overrides both, and likewise for the other passes
         // below
         const int txo = (int)mr_env_ll("MR_AHA_HIP_FO_TX", p->tile[0] > 0 ? p->tile[0] : 1 << lgT);
-        MR_TRY((mr_fft_pass_ct<false>(fo, ld, st, plane * V, 0, txo, s, "Fo")));
+        if (!two && mr_zd_ok(d, L, plane, x, S, A)) {
+            const long long nt = (plane / 16) * (L / MR_ZD_LB);
+            hipLaunchKernelGGL(mr_k_fo_dma<56>, dim3((unsigned)mr_zd_grid(nt)), dim3(1024), 0, s,
+                               fo.tws, x, S, A, plane, N, L, nc, coil0, cno, nt);
+            MR_LAUNCH_CHECK("mr_k_fo_dma");
+        } else {
+            MR_TRY((mr_fft_pass_ct<false>(fo, ld, st, plane * V, 0, txo, s, "Fo")));
+        }
         MR_TRY(mr_prof_mark(p, two ? MR_ST_F2 : MR_ST_F3));
     }
     if (!two) {
@@ -999,7 +1221,15 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
         // 32 pencils: the coil-sum accumulator makes 64 cost two workgroups per CU
         const int txi = (int)mr_env_ll("MR_AHA_HIP_IO_TX",
                                        p->tile[3] > 0 ? p->tile[3] : 1 << (lgT < 5 ? lgT : 5));
-        MR_TRY((mr_fft_pass_ct<true>(fo, ld, st, plane * L, (size_t)No, txi, s, "Io")));
+        if (!two && mr_zd_ok(d, L, plane, A, S, out)) {
+            const long long nt = (plane / 16) * (L / MR_ZD_LB);
+            hipLaunchKernelGGL(mr_k_io_dma<56>, dim3((unsigned)mr_zd_grid(nt)), dim3(1024), 0, s,
+                               fo.tws, A, S, out, plane, N, L, nc, coil0, cno, coil0 > 0 ? 1 : 0,
+                               nt);
+            MR_LAUNCH_CHECK("mr_k_io_dma");
+        } else {
+            MR_TRY((mr_fft_pass_ct<true>(fo, ld, st, plane * L, (size_t)No, txi, s, "Io")));
+        }
         MR_TRY(mr_prof_mark(p, two ? MR_ST_I2 : MR_ST_I3));
     }
     return MR_SUCCESS;

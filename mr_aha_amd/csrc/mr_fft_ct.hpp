@@ -377,21 +377,26 @@ __device__ __forceinline__ void mr_lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// Butterflies per work-item and stage, and stages, at most (lengths 512 and
-// 640 with 1024 work-items).
+// Butterflies per work-item and stage, and stages, at most (the
+// double-buffered passes: lengths 512 and 640 with 16 pencils, 56 with 128
+// pencils, 1024 work-items).
 constexpr int MR_EP_KMAX = 2;
 constexpr int MR_EP_NST = 4;
 
-// The twiddle of each of a work-item's butterflies, for every stage: the
-// same for every tile, so they are loaded once per workgroup and no vector
-// memory load (which completes in order) waits behind the next tile's rows.
-template <int N, int S, int NT, class TW>
+constexpr int mr_ep_lg(int v) { return v <= 1 ? 0 : 1 + mr_ep_lg(v / 2); }
+
+// The twiddle of each of a work-item's butterflies, for every stage of a
+// tile of PT pencils: the same for every tile, so they are loaded once per
+// workgroup and no vector memory load (which completes in order) waits
+// behind the next tile's rows.
+template <int N, int S, int NT, int PT, class TW>
 __device__ __forceinline__ void mr_ep_twload(float2 (&w)[MR_EP_NST][MR_EP_KMAX], const TW &tw)
 {
     constexpr mr_ct_fac F = mr_ct_factor(N);
     static_assert(F.rest == 1 && F.nst <= MR_EP_NST, "length must be 2^a 3^b 5^c 7^d");
+    static_assert((PT & (PT - 1)) == 0, "pencils per tile: a power of two");
     if constexpr (S < F.nst) {
-        constexpr int R = F.r[S], NS = F.ns[S], m = N / R, NB = 16 * m;
+        constexpr int R = F.r[S], NS = F.ns[S], m = N / R, NB = PT * m;
         constexpr int KMAX = (NB + NT - 1) / NT;
         static_assert(KMAX <= MR_EP_KMAX, "butterflies per work-item");
         static_assert(mr_cdiv_exact<NS>(), "inexact constant division");
@@ -400,38 +405,42 @@ __device__ __forceinline__ void mr_ep_twload(float2 (&w)[MR_EP_NST][MR_EP_KMAX],
             for (int k = 0; k < KMAX; ++k) {
                 const int f = (int)threadIdx.x + k * NT;
                 if (f < NB) {
-                    const unsigned j = (unsigned)(f >> 4);
+                    const unsigned j = (unsigned)(f >> mr_ep_lg(PT));
                     const unsigned kk = j - mr_cdiv<NS>::q(j) * NS;
                     w[S][k] = tw.get(NS + kk - 1, 0u);
                 }
             }
         }
-        mr_ep_twload<N, S + 1, NT>(w, tw);
+        mr_ep_twload<N, S + 1, NT, PT>(w, tw);
     }
 }
 
-// One Stockham stage over [element][16 pencils]: work-item f takes pencil
-// f & 15 and butterfly f >> 4, so 16 neighbouring lanes touch one row.
-// OOP: read from src, write to A (no barrier between the two phases).
-template <int N, int S, bool INV, int NT, bool OOP = false>
+// One Stockham stage over [element][PT pencils]: work-item f takes pencil
+// f mod PT and butterfly f / PT, so neighbouring lanes touch one row. OOP:
+// the inputs come from rd(pencil, element) instead of A, and no barrier
+// separates the reads from the writes.
+template <int N, int S, bool INV, int NT, int PT, bool OOP = false, class RD = int>
 __device__ __forceinline__ void mr_ep_stage(float2 *A, const float2 (&ws)[MR_EP_KMAX],
-                                            const float2 *src = nullptr)
+                                            const RD &rd = 0)
 {
     constexpr mr_ct_fac F = mr_ct_factor(N);
     constexpr int R = F.r[S], NS = F.ns[S];
     constexpr int m = N / R;
-    constexpr int NB = 16 * m;
+    constexpr int NB = PT * m;
     constexpr int KMAX = (NB + NT - 1) / NT;
     constexpr int RW = (R > 1) ? R - 1 : 1;
+    constexpr int LG = mr_ep_lg(PT);
     float2 v[KMAX][R];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         const int f = (int)threadIdx.x + k * NT;
         if (f < NB) {
-            const int p = f & 15, j = f >> 4;
-            const float2 *in = OOP ? src : A;
+            const int p = f & (PT - 1), j = f >> LG;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[k][r] = in[(j + r * m) * 16 + p];
+            for (int r = 0; r < R; ++r) {
+                if constexpr (OOP) v[k][r] = rd(p, j + r * m);
+                else v[k][r] = A[(j + r * m) * PT + p];
+            }
         }
     }
     if constexpr (!OOP) mr_lds_barrier();
@@ -439,7 +448,7 @@ __device__ __forceinline__ void mr_ep_stage(float2 *A, const float2 (&ws)[MR_EP_
     for (int k = 0; k < KMAX; ++k) {
         const int f = (int)threadIdx.x + k * NT;
         if (f < NB) {
-            const int p = f & 15, j = f >> 4;
+            const int p = f & (PT - 1), j = f >> LG;
             const unsigned jn = mr_cdiv<NS>::q((unsigned)j), kk = (unsigned)j - jn * NS;
             if constexpr (NS > 1) {
                 float2 wp[RW];
@@ -452,20 +461,20 @@ __device__ __forceinline__ void mr_ep_stage(float2 *A, const float2 (&ws)[MR_EP_
             }
             bfly<R, INV>(v[k]);
 #pragma unroll
-            for (int q = 0; q < R; ++q) A[(jn * (NS * R) + kk + q * NS) * 16 + p] = v[k][q];
+            for (int q = 0; q < R; ++q) A[(jn * (NS * R) + kk + q * NS) * PT + p] = v[k][q];
         }
     }
     mr_lds_barrier();
 }
 
 // Stages S .. last, in place.
-template <int N, int S, bool INV, int NT>
+template <int N, int S, bool INV, int NT, int PT>
 __device__ __forceinline__ void mr_ep_fft(float2 *A, const float2 (&w)[MR_EP_NST][MR_EP_KMAX])
 {
     constexpr mr_ct_fac F = mr_ct_factor(N);
     if constexpr (S < F.nst) {
-        mr_ep_stage<N, S, INV, NT>(A, w[S]);
-        mr_ep_fft<N, S + 1, INV, NT>(A, w);
+        mr_ep_stage<N, S, INV, NT, PT>(A, w[S]);
+        mr_ep_fft<N, S + 1, INV, NT, PT>(A, w);
     }
 }
 
@@ -483,7 +492,7 @@ mr_fft_axis_dma(const float2 *tws, Load ld, Store st, long long ntiles)
     __shared__ __align__(16) float2 A[16 * N];
     __shared__ __align__(16) float2 B[16 * N];
     float2 w[MR_EP_NST][MR_EP_KMAX];
-    mr_ep_twload<N, 0, NT>(w, mr_tw_buf{mr_rsrc(tws, (unsigned)N * 8u)});
+    mr_ep_twload<N, 0, NT, 16>(w, mr_tw_buf{mr_rsrc(tws, (unsigned)N * 8u)});
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int ns = RW::rows(ld), c = RW::shift(ld);
@@ -521,9 +530,9 @@ mr_fft_axis_dma(const float2 *tws, Load ld, Store st, long long ntiles)
     for (; t < ntiles; t += gridDim.x) {
         __builtin_amdgcn_s_waitcnt(0);   // this wavefront's row loads into B
         __syncthreads();                 // everyone's (and the zero rows)
-        mr_ep_stage<N, 0, INV, NT, true>(A, w[0], B);   // B -> A, ends in a barrier
+        mr_ep_stage<N, 0, INV, NT, 16, true>(A, w[0], [&](int pp, int e) { return B[e * 16 + pp]; });   // B -> A
         if (t + gridDim.x < ntiles) fill(t + gridDim.x);
-        mr_ep_fft<N, 1, INV, NT>(A, w);
+        mr_ep_fft<N, 1, INV, NT, 16>(A, w);
         const typename Store::PB sb = st.pencil(t * 16 + p, 0);
         for (int e = threadIdx.x >> 4; e < N; e += NT / 16)
             st.put(sb, p, e, 0, A[e * 16 + p], nullptr);
