@@ -378,8 +378,8 @@ __device__ __forceinline__ void mr_lds_barrier()
 }
 
 // Butterflies per work-item and stage, and stages, at most (the
-// double-buffered passes: lengths 512 and 640 with 16 pencils, 56 with 128
-// pencils, 1024 work-items).
+// double-buffered passes: lengths 320, 512 and 640 with 16 pencils, 56 with
+// 128 pencils).
 constexpr int MR_EP_KMAX = 2;
 constexpr int MR_EP_NST = 4;
 
@@ -546,7 +546,10 @@ template <int N, bool INV, class Load, class Store>
 bool mr_fft_pass_dma(const mr_fft_len<float> &f, const Load &ld, const Store &st,
                      long long npencil, hipStream_t stream)
 {
-    if constexpr (!mr_dma_rows<Load>::kOk || !mr_dma_store<Store>::kOk || N < 512) {
+    // lengths whose stages fit the registers without scratch: 320 (two
+    // workgroups of 512 per CU), 512 and 640 (one of 1024)
+    if constexpr (!mr_dma_rows<Load>::kOk || !mr_dma_store<Store>::kOk ||
+                  !(N == 320 || N >= 512)) {
         return false;
     } else {
         using RW = mr_dma_rows<Load>;
