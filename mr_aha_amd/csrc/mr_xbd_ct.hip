@@ -28,6 +28,20 @@ constexpr int mr_xbd_ct_gmax(int N, int L)
     return g < 1 ? 1 : g;
 }
 
+// Locations of a coil-pair row walked per coil instead (0: none), so that the
+// wavefronts of the block step load the four SIMDs evenly: in the 1024-thread
+// two-coil form, N/64 pair wavefronts with N/64 = 2 mod 4 leave two SIMDs one
+// walk ahead; 128 locations split into four one-coil wavefronts (half a pair
+// walk each) even them out. Needs whole 64-location blocks (the 16-byte
+// entry pairing) and 16 wavefronts.
+constexpr int mr_xbd_split_locs(int N, int NT, int GMAX, bool masked)
+{
+    return (!masked && GMAX == 2 && NT == 1024 && N % 64 == 0 && (N / 64) % 4 == 2 &&
+            N / 64 + 2 <= 16)
+               ? 128
+               : 0;
+}
+
 // Load the image points of one pencil into its N-point LDS row: source s
 // lands at e = (s - cnx) mod N. Chunks of 64 points, NC of them, issued back
 // to back; lanes past the Nx points (last chunk) read zero onto padding slots.
@@ -230,7 +244,35 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
                 mr_bd_walk_g<L, false>(a0, N, ker, kstride, (unsigned)(b * 8));
             }
         };
-        if constexpr (((GMAX + 1) / 2) * N <= NT) {
+        if constexpr (((GMAX + 1) / 2) * N <= NT && mr_xbd_split_locs(N, NT, GMAX, MASKED) > 0) {
+            // N/64 pair wavefronts would load the four SIMDs unevenly (N = 640:
+            // 3, 3, 2, 2 walks). The last NS locations are walked per coil
+            // instead, by 2 NS / 64 more wavefronts: wavefronts 0 .. NP/64 - 1
+            // walk coil pairs, then coil 0 and coil 1 of locations NP .. N - 1,
+            // so each SIMD holds the same share of FMAs (N = 640: two pair
+            // walks and one one-coil walk, about half as long). The one-coil
+            // wavefronts load the same entries as each other, mostly from L2.
+            constexpr int NS = mr_xbd_split_locs(N, NT, GMAX, MASKED);
+            constexpr int NP = N - NS;
+            const int t = threadIdx.x;
+            const bool split = x2 && gn == 2;
+            if (t < (split ? NP : nitem)) {
+                item(t);
+            } else if (split && t < NP + 2 * NS) {
+                const int u = t - NP;
+                const int g = u / NS;
+                int e = NP + (u - g * NS);
+                const int hi = (e & 63) >> 5;
+                e = (e & ~63) | (((e & 31) << 1) + hi);
+                const long long b = kidx(e);
+                float2 *a = A + g * L * N + e;
+                const unsigned bp = (unsigned)((b - hi) * 8 + (hi ? kstride * 8 : 0));
+                if (d.tri_upper)
+                    mr_bd_walk_pair_x2<L, true, 8, 1>(a, a, N, ker, kstride, bp, (unsigned)(b * 8));
+                else
+                    mr_bd_walk_pair_x2<L, false, 8, 1>(a, a, N, ker, kstride, bp, (unsigned)(b * 8));
+            }
+        } else if constexpr (((GMAX + 1) / 2) * N <= NT) {
             // a thread per item: one round, no loop (a loop here keeps the
             // walk's column bases live across iterations and spills)
             if ((int)threadIdx.x < nitem) item(threadIdx.x);

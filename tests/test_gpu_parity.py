@@ -427,10 +427,15 @@ def test_nufft_method(mr, golden, name):
 
 # fused x pass variants: pencil I/O one point per lane (x length 2 mod 4) or
 # two (0 mod 4); kernel entries two per 16-byte load (16-byte aligned kernel)
-# or one per 8-byte load (kernel pointer offset by 8 bytes)
+# or one per 8-byte load (kernel pointer offset by 8 bytes); at x length 640
+# in the 1024-thread two-coil form, the last 128 locations of an aligned
+# coil pair walked per coil
 FUSED_PATHS = [((126, 10), (256, 20), 16, 2, False), ((250, 8), (640, 16), 16, 2, False),
                ((160, 8), (320, 16), 16, 3, True), ((320, 12), (640, 24), 16, 2, True),
-               ((96, 12), (384, 24), 8, 3, True), ((320, 12), (640, 24), 16, 1, False)]
+               ((96, 12), (384, 24), 8, 3, True), ((320, 12), (640, 24), 16, 1, False),
+               # x length 640 in coil pairs with an odd last coil: the pair group
+               # walks its last 128 locations per coil, the lone coil does not
+               ((320, 12), (640, 24), 16, 3, False)]
 
 
 @pytest.mark.parametrize("wide", [True, False])
