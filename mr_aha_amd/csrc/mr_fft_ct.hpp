@@ -111,9 +111,18 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
     constexpr int KMAX = (TPMAX * m + NT - 1) / NT;
     constexpr int RW = (R > 1) ? R - 1 : 1;
     static_assert(mr_cdiv_exact<m>() && mr_cdiv_exact<NS>(), "inexact constant division");
+    // Butterfly f uses twiddle index (f mod m) mod NS = f mod NS (NS divides
+    // m). With NS dividing NT, that is threadIdx.x mod NS in every round of a
+    // thread: its twiddles are read, and their powers formed, once (SH).
+    // Only a clamped lane (f past nb - 1 in a later round) would then
+    // multiply with another butterfly's twiddles; it stores nothing. A
+    // wavefront whose first round has clamped lanes has no later round
+    // (nb <= 64 wave + 63 < NT + 64 wave), so its twiddles are its own.
+    constexpr bool SH = NS > 1 && KMAX > 1 && NT % NS == 0;
+    constexpr int KW = SH ? 1 : KMAX;
     const int nb = tp * m;
     float2 v[KMAX][R];
-    float2 w[KMAX][TW::kPowers ? 1 : RW];
+    float2 w[KW][TW::kPowers ? 1 : RW];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         if (k * NT + wave * 64 < nb) {
@@ -129,25 +138,42 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                 for (int r = 0; r < R; ++r) v[k][r] = a[r * m];
             }
             if constexpr (NS > 1) {
-                const unsigned kk = j - mr_cdiv<NS>::q(j) * NS;
-                if constexpr (TW::kPowers) {
-                    w[k][0] = tw.get(NS + kk - 1, 0u);
-                } else {
+                if (k < KW) {
+                    const unsigned kk = j - mr_cdiv<NS>::q(j) * NS;
+                    if constexpr (TW::kPowers) {
+                        w[k][0] = tw.get(NS + kk - 1, 0u);
+                    } else {
 #pragma unroll
-                    for (int r = 1; r < R; ++r)
-                        w[k][TW::kPowers ? 0 : r - 1] = tw.get(NS + kk - 1, (unsigned)((r - 1) * NS));
+                        for (int r = 1; r < R; ++r)
+                            w[k][r - 1] = tw.get(NS + kk - 1, (unsigned)((r - 1) * NS));
+                    }
                 }
             }
         }
     }
     __syncthreads();
+    // twiddle powers W^r, r = 1 .. R-1, of a shared twiddle: each a product of
+    // two lower ones (the same products, in the same order, as per round)
+    float2 ws[SH && TW::kPowers ? RW : 1];
+    if constexpr (SH && TW::kPowers) {
+        if (wave * 64 < nb) {
+            ws[0] = w[0][0];
+#pragma unroll
+            for (int r = 2; r < R; ++r) ws[r - 1] = cmul(ws[r / 2 - 1], ws[r - r / 2 - 1]);
+        }
+    }
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         if (k * NT + wave * 64 < nb) {
-            const unsigned f = min((unsigned)(threadIdx.x + k * NT), (unsigned)(nb - 1));
+            const unsigned fr = threadIdx.x + k * NT;
+            const unsigned f = min(fr, (unsigned)(nb - 1));
             const unsigned p = mr_cdiv<m>::q(f), j = f - p * m;
             const unsigned jn = mr_cdiv<NS>::q(j), kk = j - jn * NS;
-            if constexpr (NS > 1 && TW::kPowers) {
+            if constexpr (NS > 1 && SH && TW::kPowers) {
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    v[k][r] = INV ? cmulc(ws[r - 1], v[k][r]) : cmul(ws[r - 1], v[k][r]);
+            } else if constexpr (NS > 1 && TW::kPowers) {
                 // each power applied as soon as it is formed, so few are live
                 float2 wp[RW];
                 wp[0] = w[k][0];
@@ -157,20 +183,24 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                     v[k][r] = INV ? cmulc(wp[r - 1], v[k][r]) : cmul(wp[r - 1], v[k][r]);
                 }
             } else if constexpr (NS > 1) {
+                constexpr int kw = 0;
 #pragma unroll
                 for (int r = 1; r < R; ++r)
-                    v[k][r] = INV ? cmulc(w[k][TW::kPowers ? 0 : r - 1], v[k][r])
-                                  : cmul(w[k][TW::kPowers ? 0 : r - 1], v[k][r]);
+                    v[k][r] = INV ? cmulc(w[SH ? kw : k][r - 1], v[k][r])
+                                  : cmul(w[SH ? kw : k][r - 1], v[k][r]);
             }
             bfly<R, INV>(v[k]);
-            float2 *b = A + p * NSTR + jn * (NS * R) + kk;
-            if constexpr (SWO) {
-                const unsigned c = mr_ct_swz(p), o = jn * (NS * R) + kk;
+            // a clamped lane of a shared-twiddle stage holds wrong values
+            if (!SH || k == 0 || fr < (unsigned)nb) {
+                float2 *b = A + p * NSTR + jn * (NS * R) + kk;
+                if constexpr (SWO) {
+                    const unsigned c = mr_ct_swz(p), o = jn * (NS * R) + kk;
 #pragma unroll
-                for (int q = 0; q < R; ++q) A[p * NSTR + ((o + q * NS) ^ c)] = v[k][q];
-            } else {
+                    for (int q = 0; q < R; ++q) A[p * NSTR + ((o + q * NS) ^ c)] = v[k][q];
+                } else {
 #pragma unroll
-                for (int q = 0; q < R; ++q) b[q * NS] = v[k][q];
+                    for (int q = 0; q < R; ++q) b[q * NS] = v[k][q];
+                }
             }
         }
     }
