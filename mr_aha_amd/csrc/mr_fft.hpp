@@ -190,13 +190,10 @@ template <bool INV, typename V> __device__ __forceinline__ void bfly4(V *v)
     v[3] = csub(b, d);
 }
 
-template <bool INV, typename V> __device__ __forceinline__ void bfly8(V *v)
+// bfly8 from the radix-4 transforms e of the even and o of the odd inputs.
+template <bool INV, typename V> __device__ __forceinline__ void bfly8_join(V *v, const V *e, const V *o)
 {
     using R = decltype(V::x);
-    V e[4] = {v[0], v[2], v[4], v[6]};
-    V o[4] = {v[1], v[3], v[5], v[7]};
-    bfly4<INV>(e);
-    bfly4<INV>(o);
     const R h = (R)0.70710678118654752440;
     // w8^q for q = 0..3: 1, (1 -+ i)/sqrt2, -+i, (-1 -+ i)/sqrt2
     V t1 = INV ? cmake<V>(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y))
@@ -208,6 +205,36 @@ template <bool INV, typename V> __device__ __forceinline__ void bfly8(V *v)
     v[1] = cadd(e[1], t1);    v[5] = csub(e[1], t1);
     v[2] = cadd(e[2], t2);    v[6] = csub(e[2], t2);
     v[3] = cadd(e[3], t3);    v[7] = csub(e[3], t3);
+}
+
+template <bool INV, typename V> __device__ __forceinline__ void bfly8(V *v)
+{
+    V e[4] = {v[0], v[2], v[4], v[6]};
+    V o[4] = {v[1], v[3], v[5], v[7]};
+    bfly4<INV>(e);
+    bfly4<INV>(o);
+    bfly8_join<INV>(v, e, o);
+}
+
+// bfly4 of (a, 0, 0, b): a +- b and a +- (-+i)(-b).
+template <bool INV, typename V> __device__ __forceinline__ void bfly4_ends(V *r, V a, V b)
+{
+    const V d = cmul_mi<INV>(cmake<V>(-b.x, -b.y));
+    r[0] = cadd(a, b);
+    r[2] = csub(a, b);
+    r[1] = cadd(a, d);
+    r[3] = csub(a, d);
+}
+
+// bfly8 with v[2] .. v[5] zero (the first stage of a zero-padded centred
+// pencil): the same values as bfly8 up to the sign of an exact zero, with
+// half the additions of its radix-4 halves.
+template <bool INV, typename V> __device__ __forceinline__ void bfly8_ends(V *v)
+{
+    V e[4], o[4];
+    bfly4_ends<INV>(e, v[0], v[6]);
+    bfly4_ends<INV>(o, v[1], v[7]);
+    bfly8_join<INV>(v, e, o);
 }
 
 // Odd prime radix with compile-time constants: V_q = sum_r v_r e^{-+2 pi i r q/R},

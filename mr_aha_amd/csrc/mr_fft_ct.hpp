@@ -103,10 +103,18 @@ struct mr_tw_lds {
 // pencils of one row on the same LDS banks); inner stages use plain order.
 __device__ __forceinline__ unsigned mr_ct_swz(unsigned p) { return (p & 7u) << 2; }
 
+// PR = 1: the first forward stage of a centred zero-padded pencil (image
+// points at e < N/4 and e >= 3N/4, zeros between; radix 8): inputs r = 2..5
+// are zero, so they are not read, and the butterfly skips them. PR = 2: the
+// last inverse stage when only outputs e < N/4 and e >= 3N/4 are kept (the
+// crop of the same pencil): outputs landing wholly between are not stored,
+// and their arithmetic is dropped with them.
 template <int N, int NSTR, int R, int NS, bool INV, int NT, int TPMAX, class TW, bool SWI = false,
-          bool SWO = false>
+          bool SWO = false, int PR = 0>
 __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int wave)
 {
+    static_assert(PR != 1 || (R == 8 && NS == 1 && !INV && !SWI), "padded first stage");
+    static_assert(PR != 2 || (NS * R == N && INV && !SWO && N % 4 == 0), "cropped last stage");
     constexpr int m = N / R;
     constexpr int KMAX = (TPMAX * m + NT - 1) / NT;
     constexpr int RW = (R > 1) ? R - 1 : 1;
@@ -133,6 +141,11 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                 const unsigned c = mr_ct_swz(p);
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[k][r] = A[p * NSTR + ((j + r * m) ^ c)];
+            } else if constexpr (PR == 1) {
+                v[k][0] = a[0];
+                v[k][1] = a[m];
+                v[k][6] = a[6 * m];
+                v[k][7] = a[7 * m];
             } else {
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[k][r] = a[r * m];
@@ -189,7 +202,8 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                     v[k][r] = INV ? cmulc(w[SH ? kw : k][r - 1], v[k][r])
                                   : cmul(w[SH ? kw : k][r - 1], v[k][r]);
             }
-            bfly<R, INV>(v[k]);
+            if constexpr (PR == 1) bfly8_ends<INV>(v[k]);
+            else bfly<R, INV>(v[k]);
             // a clamped lane of a shared-twiddle stage holds wrong values
             if (!SH || k == 0 || fr < (unsigned)nb) {
                 float2 *b = A + p * NSTR + jn * (NS * R) + kk;
@@ -199,7 +213,9 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                     for (int q = 0; q < R; ++q) A[p * NSTR + ((o + q * NS) ^ c)] = v[k][q];
                 } else {
 #pragma unroll
-                    for (int q = 0; q < R; ++q) b[q * NS] = v[k][q];
+                    for (int q = 0; q < R; ++q)
+                        if (PR != 2 || NS * q < N / 4 || NS * (q + 1) > 3 * (N / 4))
+                            b[q * NS] = v[k][q];
                 }
             }
         }
@@ -207,16 +223,39 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
     __syncthreads();
 }
 
-template <int N, int NSTR, int S, bool INV, int NT, int TPMAX, class TW, bool SW = false>
+// Stages S .. SE - 1 of the transform (SE = 99: to the last).
+template <int N, int NSTR, int S, bool INV, int NT, int TPMAX, class TW, bool SW = false, int SE = 99>
 __device__ __forceinline__ void mr_ct_fft(float2 *A, const TW &tw, int tp, int wave)
 {
     constexpr mr_ct_fac F = mr_ct_factor(N);
     static_assert(F.rest == 1, "length must be 2^a 3^b 5^c 7^d");
     static_assert(!SW || (N % 32 == 0 && F.nst >= 2), "swizzled order needs 32 | N, 2 stages");
-    if constexpr (S < F.nst) {
+    if constexpr (S < F.nst && S < SE) {
         mr_ct_stage<N, NSTR, F.r[S], F.ns[S], INV, NT, TPMAX, TW, SW && S == 0,
                     SW && S == F.nst - 1>(A, tw, tp, wave);
-        mr_ct_fft<N, NSTR, S + 1, INV, NT, TPMAX, TW, SW>(A, tw, tp, wave);
+        mr_ct_fft<N, NSTR, S + 1, INV, NT, TPMAX, TW, SW, SE>(A, tw, tp, wave);
+    }
+}
+
+// Transform of centred zero-padded pencils whose result is cropped the same
+// way (image points at e < N/4 and e >= 3N/4): forward with the padded first
+// stage, inverse with the cropped last stage; plain order, first radix 8.
+template <int N> constexpr bool mr_ct_can_trim()
+{
+    return mr_ct_factor(N).r[0] == 8 && N % 32 == 0 && mr_ct_factor(N).nst >= 2;
+}
+template <int N, int NSTR, bool INV, int NT, int TPMAX, class TW>
+__device__ __forceinline__ void mr_ct_fft_trim(float2 *A, const TW &tw, int tp, int wave)
+{
+    constexpr mr_ct_fac F = mr_ct_factor(N);
+    static_assert(mr_ct_can_trim<N>(), "trimmed transform");
+    constexpr int L = F.nst - 1;
+    if constexpr (!INV) {
+        mr_ct_stage<N, NSTR, F.r[0], F.ns[0], false, NT, TPMAX, TW, false, false, 1>(A, tw, tp, wave);
+        mr_ct_fft<N, NSTR, 1, false, NT, TPMAX, TW>(A, tw, tp, wave);
+    } else {
+        mr_ct_fft<N, NSTR, 0, true, NT, TPMAX, TW, false, L>(A, tw, tp, wave);
+        mr_ct_stage<N, NSTR, F.r[L], F.ns[L], true, NT, TPMAX, TW, false, false, 2>(A, tw, tp, wave);
     }
 }
 

@@ -145,6 +145,10 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
     // two points per lane when the pencils and their slots pair up (Nx a
     // multiple of 4, so Nx and cnx = Nx/2 are even; 16-byte aligned rows)
     const bool px = (Nx & 3) == 0 && ((unsigned long long)Y & 15) == 0;
+    // the image fills e < N/4 and e >= 3N/4 exactly (N = 2 Nx, centre Nx/2):
+    // the transforms skip the zeros between and the cropped outputs, and the
+    // padding slots, which the first stage then never reads, stay unwritten
+    const bool trim = mr_ct_can_trim<N>() && 2 * Nx == N && 4 * cnx == N;
     for (int pl = wave; pl < tp; pl += NW) {
         const __amdgpu_buffer_rsrc_t rs = mr_rsrc(Yr + pstride * pl, rowbytes);
         float2 *Ap = A + pl * N;
@@ -152,15 +156,23 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
             mr_xbd_load_src2<N>(Ap, rs, lane, cnx, Nx, (Nx + 127) >> 7);
         else
             mr_xbd_load_src<N>(Ap, rs, lane, cnx, nck);
-        for (int i = lane; i < N - Nx; i += 64) {
-            int e = Nx + i - cnx;
-            if (e >= N) e -= N;
-            if (e < 0) e += N;
-            Ap[e] = make_float2(0.f, 0.f);
-        }
+        if (!trim)
+            for (int i = lane; i < N - Nx; i += 64) {
+                int e = Nx + i - cnx;
+                if (e >= N) e -= N;
+                if (e < 0) e += N;
+                Ap[e] = make_float2(0.f, 0.f);
+            }
     }
     __syncthreads();
-    mr_ct_fft<N, N, 0, false, NT, TPMAX>(A, twb, tp, wave);
+    if constexpr (mr_ct_can_trim<N>()) {
+        if (trim)
+            mr_ct_fft_trim<N, N, false, NT, TPMAX>(A, twb, tp, wave);
+        else
+            mr_ct_fft<N, N, 0, false, NT, TPMAX>(A, twb, tp, wave);
+    } else {
+        mr_ct_fft<N, N, 0, false, NT, TPMAX>(A, twb, tp, wave);
+    }
 
     const int My = d.m[1];
     const int jy = (int)(r % My), jz = (int)(r / My);
@@ -313,7 +325,14 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
     }
     }
     __syncthreads();
-    mr_ct_fft<N, N, 0, true, NT, TPMAX>(A, twb, tp, wave);
+    if constexpr (mr_ct_can_trim<N>()) {
+        if (trim)
+            mr_ct_fft_trim<N, N, true, NT, TPMAX>(A, twb, tp, wave);
+        else
+            mr_ct_fft<N, N, 0, true, NT, TPMAX>(A, twb, tp, wave);
+    } else {
+        mr_ct_fft<N, N, 0, true, NT, TPMAX>(A, twb, tp, wave);
+    }
 
     for (int pl = wave; pl < tp; pl += NW) {
         const __amdgpu_buffer_rsrc_t rs = mr_rsrc(Yr + pstride * pl, rowbytes);
