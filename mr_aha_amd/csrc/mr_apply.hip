@@ -564,6 +564,7 @@ struct mr_aha_plan_s {
     int tile[4];          // pencils per workgroup of Fo, Fy, Iy, Io (0: default rule)
     int no_wide;          // 1: no two-coil 1024-thread fused workgroups
     mr_xbd_ct_fn xbd_mf;  // xbd_ct with the block step on the matrix cores, or null
+    mr_xbd_ct_fn xbd_pd;  // persistent form of xbd_ct (one workgroup per CU), or null
     int no_mf;            // 1: keep the FMA block walks (built-in rule; the autotuner may pick xbd_mf)
     bool tuning;          // mr_aha_plan_autotune is timing stages
     int kf;               // kernel format used by execute (MR_KF_*)
@@ -673,6 +674,7 @@ static int mr_xbd_setup(mr_aha_plan p)
     p->xbd_lds = 0;
     p->xbd_ct = nullptr;
     p->xbd_mf = nullptr;
+    p->xbd_pd = nullptr;
     p->xbd_nt = MR_XBD_THREADS;
     const mr_config &cfg = p->cfg;
     const int L = cfg.L, Mx = p->dims.m[0];
@@ -732,6 +734,20 @@ static int mr_xbd_setup(mr_aha_plan p)
         (void)hipGetLastError();
         return MR_SUCCESS;
     }
+    // Persistent form of the wide workgroups: the image fills the centred
+    // half of the x grid (Mx = 2 Nx, centre Nx/2) and its rows are whole
+    // 16-byte units, so the next row's image can stream into the LDS slots
+    // the current row's result does not occupy
+    mr_xbd_ct_fn pd = nullptr;
+    if (nt == 1024 && 2 * p->dims.n[0] == Mx && 4 * p->dims.c_n[0] == Mx &&
+        p->dims.n[0] % 4 == 0 && mr_env_ll("MR_AHA_HIP_XBD_PD", 1) != 0) {
+        pd = mr_xbd_ct_pick_persistent(Mx, L, cfg.is_masked != 0, p->kf);
+        if (pd && hipFuncSetAttribute((const void *)pd, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds) != hipSuccess) {
+            (void)hipGetLastError();
+            pd = nullptr;
+        }
+    }
     // The same kernel with its block step on the matrix cores: FP32 kernel,
     // unmasked, L a multiple of 4 and an even kx centering shift (adjacent
     // locations then have adjacent kernel entries). It runs when
@@ -755,6 +771,7 @@ static int mr_xbd_setup(mr_aha_plan p)
     p->xbd_lds = lds;
     p->xbd_ct = ct;
     p->xbd_mf = mf;
+    p->xbd_pd = pd;
     p->xbd_nt = nt;
     p->nxdiv = mr_fdiv_make((unsigned)p->dims.n[0]);
     return MR_SUCCESS;
@@ -1192,10 +1209,12 @@ static int mr_execute_xlast(mr_aha_plan p, const float2 *x, const float2 *S, con
         if (p->xbd_ct) {
             // the matrix-core block step reads kernel entries two locations
             // per 16-byte load
-            const mr_xbd_ct_fn fn = (p->xbd_mf && !p->no_mf && ((unsigned long long)ker & 15) == 0)
-                                        ? p->xbd_mf
-                                        : p->xbd_ct;
-            hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3((unsigned)p->xbd_nt), p->xbd_lds,
+            const bool mf = p->xbd_mf && !p->no_mf && ((unsigned long long)ker & 15) == 0;
+            // the persistent form walks all items with one workgroup per CU
+            const bool pd = !mf && p->xbd_pd && ((unsigned long long)B & 15) == 0;
+            const mr_xbd_ct_fn fn = mf ? p->xbd_mf : pd ? p->xbd_pd : p->xbd_ct;
+            const long long grid = pd ? mr_zd_grid(blocks) : blocks;
+            hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3((unsigned)p->xbd_nt), p->xbd_lds,
                                s, B, ker, p->d_inv, kstride, d, p->fx.tws, R, G, nc, ncg, p->kq,
                                p->ksc);
             MR_LAUNCH_CHECK("mr_k_xbd_ct");

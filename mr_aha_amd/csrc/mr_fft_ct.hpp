@@ -91,6 +91,30 @@ struct mr_tw_lds {
     __device__ __forceinline__ float2 get(unsigned idx, unsigned off) const { return t[idx + off]; }
 };
 
+// Workgroup barrier over LDS traffic only: this wavefront's LDS operations
+// complete, then s_barrier. __syncthreads() would also drain the row loads
+// still streaming into the other buffer (its release fence covers every LDS
+// write, and a global-to-LDS load is one), so the overlap would be lost. No
+// memory access moves across it (memory clobber); the row loads are awaited
+// explicitly where their buffer is read.
+__device__ __forceinline__ void mr_lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// No hook between a stage's reads and its writes (mr_ct_stage).
+struct mr_ct_nomid {
+    static constexpr bool active = false;
+    __device__ void operator()() const {}
+};
+// A hook run once a stage's inputs and twiddles are in registers, before its
+// writes: work that may reuse the LDS slots the stage does not write.
+template <class F> struct mr_ct_mid {
+    static constexpr bool active = true;
+    F f;
+    __device__ void operator()() const { f(); }
+};
+
 // One Stockham stage (radix R, Ns = NS) in place over tp pencils of length N
 // at pencil stride NSTR in LDS, NT threads. Phase 1 reads every butterfly
 // input and twiddle of the thread into registers, phase 2 (after the
@@ -110,11 +134,25 @@ __device__ __forceinline__ unsigned mr_ct_swz(unsigned p) { return (p & 7u) << 2
 // crop of the same pencil): outputs landing wholly between are not stored,
 // and their arithmetic is dropped with them.
 template <int N, int NSTR, int R, int NS, bool INV, int NT, int TPMAX, class TW, bool SWI = false,
-          bool SWO = false, int PR = 0>
-__device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int wave)
+          bool SWO = false, int PR = 0, bool LB = false, class MID = mr_ct_nomid>
+__device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int wave,
+                                            const MID &mid = MID{})
 {
-    static_assert(PR != 1 || (R == 8 && NS == 1 && !INV && !SWI), "padded first stage");
-    static_assert(PR != 2 || (NS * R == N && INV && !SWO && N % 4 == 0), "cropped last stage");
+    static_assert((PR != 1 && PR != 4) || (R == 8 && NS == 1 && !INV && !SWI),
+                  "padded first stage");
+    static_assert((PR != 2 && PR != 3) || (NS * R == N && INV && !SWO && N % 4 == 0),
+                  "cropped last stage");
+    // LB: barriers wait for this wavefront's LDS traffic only, not for its
+    // global loads and stores (global-to-LDS loads may be in flight)
+    auto barrier = [] {
+        if constexpr (LB) mr_lds_barrier();
+        else __syncthreads();
+    };
+    // with LB (a persistent loop), the thread index is hidden behind an empty
+    // asm so that the addressing derived from it is recomputed per stage
+    // instead of held in registers across the loop
+    unsigned tid = threadIdx.x;
+    if constexpr (LB) asm volatile("" : "+v"(tid));
     constexpr int m = N / R;
     constexpr int KMAX = (TPMAX * m + NT - 1) / NT;
     constexpr int RW = (R > 1) ? R - 1 : 1;
@@ -134,7 +172,7 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         if (k * NT + wave * 64 < nb) {
-            const unsigned f = min((unsigned)(threadIdx.x + k * NT), (unsigned)(nb - 1));
+            const unsigned f = min((tid + k * NT), (unsigned)(nb - 1));
             const unsigned p = mr_cdiv<m>::q(f), j = f - p * m;
             const float2 *a = A + p * NSTR + j;
             if constexpr (SWI) {
@@ -146,6 +184,13 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                 v[k][1] = a[m];
                 v[k][6] = a[6 * m];
                 v[k][7] = a[7 * m];
+            } else if constexpr (PR == 4) {
+                // the image stored half a pencil on (e + N/2 mod N): input r
+                // of the butterfly sits at slot (r + 4 mod 8) m
+                v[k][0] = a[4 * m];
+                v[k][1] = a[5 * m];
+                v[k][6] = a[2 * m];
+                v[k][7] = a[3 * m];
             } else {
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[k][r] = a[r * m];
@@ -164,7 +209,7 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
             }
         }
     }
-    __syncthreads();
+    barrier();
     // twiddle powers W^r, r = 1 .. R-1, of a shared twiddle: each a product of
     // two lower ones (the same products, in the same order, as per round)
     float2 ws[SH && TW::kPowers ? RW : 1];
@@ -175,10 +220,18 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
             for (int r = 2; r < R; ++r) ws[r - 1] = cmul(ws[r / 2 - 1], ws[r - r / 2 - 1]);
         }
     }
+    if constexpr (MID::active) {
+        // the twiddle loads complete before the hook: a wait for them after
+        // it would also wait for whatever memory traffic the hook starts
+        static_assert(SH && TW::kPowers, "stage hook needs the shared twiddle form");
+#pragma unroll
+        for (int r = 0; r < RW; ++r) asm volatile("" : "+v"(ws[r].x), "+v"(ws[r].y));
+        mid();
+    }
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         if (k * NT + wave * 64 < nb) {
-            const unsigned fr = threadIdx.x + k * NT;
+            const unsigned fr = tid + k * NT;
             const unsigned f = min(fr, (unsigned)(nb - 1));
             const unsigned p = mr_cdiv<m>::q(f), j = f - p * m;
             const unsigned jn = mr_cdiv<NS>::q(j), kk = j - jn * NS;
@@ -202,7 +255,7 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                     v[k][r] = INV ? cmulc(w[SH ? kw : k][r - 1], v[k][r])
                                   : cmul(w[SH ? kw : k][r - 1], v[k][r]);
             }
-            if constexpr (PR == 1) bfly8_ends<INV>(v[k]);
+            if constexpr (PR == 1 || PR == 4) bfly8_ends<INV>(v[k]);
             else bfly<R, INV>(v[k]);
             // a clamped lane of a shared-twiddle stage holds wrong values
             if (!SH || k == 0 || fr < (unsigned)nb) {
@@ -213,18 +266,30 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                     for (int q = 0; q < R; ++q) A[p * NSTR + ((o + q * NS) ^ c)] = v[k][q];
                 } else {
 #pragma unroll
-                    for (int q = 0; q < R; ++q)
-                        if (PR != 2 || NS * q < N / 4 || NS * (q + 1) > 3 * (N / 4))
+                    for (int q = 0; q < R; ++q) {
+                        constexpr int Q4 = N / 4;
+                        const bool in = NS * q + NS <= Q4 || NS * q >= 3 * Q4;
+                        const bool out = NS * q >= Q4 && NS * q + NS <= 3 * Q4;
+                        if constexpr (PR == 3) {
+                            // exactly the kept slots: the others may be in use
+                            const unsigned e = kk + NS * q;
+                            if (in || (!out && (e < (unsigned)Q4 || e >= 3u * Q4)))
+                                b[q * NS] = v[k][q];
+                        } else if (PR != 2 || !out) {
                             b[q * NS] = v[k][q];
+                        }
+                    }
                 }
             }
         }
     }
-    __syncthreads();
+    barrier();
 }
 
-// Stages S .. SE - 1 of the transform (SE = 99: to the last).
-template <int N, int NSTR, int S, bool INV, int NT, int TPMAX, class TW, bool SW = false, int SE = 99>
+// Stages S .. SE - 1 of the transform (SE = 99: to the last); LB as in
+// mr_ct_stage.
+template <int N, int NSTR, int S, bool INV, int NT, int TPMAX, class TW, bool SW = false, int SE = 99,
+          bool LB = false>
 __device__ __forceinline__ void mr_ct_fft(float2 *A, const TW &tw, int tp, int wave)
 {
     constexpr mr_ct_fac F = mr_ct_factor(N);
@@ -232,8 +297,8 @@ __device__ __forceinline__ void mr_ct_fft(float2 *A, const TW &tw, int tp, int w
     static_assert(!SW || (N % 32 == 0 && F.nst >= 2), "swizzled order needs 32 | N, 2 stages");
     if constexpr (S < F.nst && S < SE) {
         mr_ct_stage<N, NSTR, F.r[S], F.ns[S], INV, NT, TPMAX, TW, SW && S == 0,
-                    SW && S == F.nst - 1>(A, tw, tp, wave);
-        mr_ct_fft<N, NSTR, S + 1, INV, NT, TPMAX, TW, SW, SE>(A, tw, tp, wave);
+                    SW && S == F.nst - 1, 0, LB>(A, tw, tp, wave);
+        mr_ct_fft<N, NSTR, S + 1, INV, NT, TPMAX, TW, SW, SE, LB>(A, tw, tp, wave);
     }
 }
 
@@ -257,6 +322,32 @@ __device__ __forceinline__ void mr_ct_fft_trim(float2 *A, const TW &tw, int tp, 
         mr_ct_fft<N, NSTR, 0, true, NT, TPMAX, TW, false, L>(A, tw, tp, wave);
         mr_ct_stage<N, NSTR, F.r[L], F.ns[L], true, NT, TPMAX, TW, false, false, 2>(A, tw, tp, wave);
     }
+}
+
+// The same for pencils whose image is stored half a pencil on, in slots
+// [N/4, 3N/4) (so that the kept slots [0, N/4) and [3N/4, N) of the previous
+// row's result and the next row's image coexist): the forward transform reads
+// the image there; the inverse writes exactly the kept slots and runs mid()
+// once its last stage has read its inputs, when the other slots are free.
+// Barriers wait for LDS traffic only.
+template <int N, int NSTR, int NT, int TPMAX, class TW>
+__device__ __forceinline__ void mr_ct_fft_mid_fwd(float2 *A, const TW &tw, int tp, int wave)
+{
+    constexpr mr_ct_fac F = mr_ct_factor(N);
+    static_assert(mr_ct_can_trim<N>(), "trimmed transform");
+    mr_ct_stage<N, NSTR, F.r[0], F.ns[0], false, NT, TPMAX, TW, false, false, 4, true>(A, tw, tp, wave);
+    mr_ct_fft<N, NSTR, 1, false, NT, TPMAX, TW, false, 99, true>(A, tw, tp, wave);
+}
+template <int N, int NSTR, int NT, int TPMAX, class TW, class MID>
+__device__ __forceinline__ void mr_ct_fft_crop_inv(float2 *A, const TW &tw, int tp, int wave,
+                                                   const MID &mid)
+{
+    constexpr mr_ct_fac F = mr_ct_factor(N);
+    static_assert(mr_ct_can_trim<N>(), "trimmed transform");
+    constexpr int L = F.nst - 1;
+    mr_ct_fft<N, NSTR, 0, true, NT, TPMAX, TW, false, L, true>(A, tw, tp, wave);
+    mr_ct_stage<N, NSTR, F.r[L], F.ns[L], true, NT, TPMAX, TW, false, false, 3, true, MID>(
+        A, tw, tp, wave, mid);
 }
 
 // Axis pass of the FFT engine at compile-time length N (same functor
@@ -435,16 +526,6 @@ template <class Store> struct mr_dma_store { static constexpr bool kOk = false; 
 template <> struct mr_dma_store<mr_st_axis<float2, false>> { static constexpr bool kOk = true; };
 template <> struct mr_dma_store<mr_st_crop<float2, false>> { static constexpr bool kOk = true; };
 
-// Workgroup barrier over LDS traffic only: this wavefront's LDS operations
-// complete, then s_barrier. __syncthreads() would also drain the row loads
-// still streaming into the other buffer (its release fence covers every LDS
-// write, and a global-to-LDS load is one), so the overlap would be lost. No
-// memory access moves across it (memory clobber); the row loads are awaited
-// explicitly where their buffer is read.
-__device__ __forceinline__ void mr_lds_barrier()
-{
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
 
 // Butterflies per work-item and stage, and stages, at most (the
 // double-buffered passes: lengths 320, 512 and 640 with 16 pencils, 56 with

@@ -624,6 +624,10 @@ mr_xbd_ct_fn mr_xbd_ct_pick(int n, int L, bool masked, int kf, int *gmax, bool m
 // The workgroup of 1024 threads and two coils (160 KB of LDS, one workgroup
 // per CU) for x length n, rank L and kernel format kf, or nullptr.
 mr_xbd_ct_fn mr_xbd_ct_pick_wide(int n, int L, bool masked, int kf, bool mf = false);
+// Persistent two-coil 1024-thread form (x length 512 or 640, L = 16, FP32
+// kernel), for an image filling the centred half of the x grid with 16-byte
+// aligned rows; launched with one workgroup per CU. Null for other shapes.
+mr_xbd_ct_fn mr_xbd_ct_pick_persistent(int n, int L, bool masked, int kf);
 
 // Fill the 16-bit format q [T, kcol], sc [NB, kcol] from an FP32 kernel.
 int mr_kf_quantize(int kf, const float2 *ker, long long kcol, int L, int upper, unsigned *q,

@@ -106,74 +106,19 @@ __device__ __forceinline__ void mr_xbd_load_src2(float2 *Ap, __amdgpu_buffer_rsr
     }
 }
 
-// NT threads, up to GMAX coils per workgroup (the default: two workgroups of
-// 512 threads per CU; the wide form: 1024 threads and two coils in 160 KB,
-// one workgroup per CU). Either way a CU holds 16 wavefronts.
-// MF: the block step on the matrix cores (mr_bd_mfma32; FP32 kernel,
-// unmasked, 16-byte aligned kernel), a wavefront per 32 locations of a coil
-// pair.
-template <int N, int L, bool MASKED, int KF, int NT = MR_XBD_THREADS,
-          int GMAX = mr_xbd_ct_gmax(N, L), bool MF = false>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
-mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__restrict__ inv,
-            long long kstride, mr_dims d, const float2 *tws, long long R, int G, int ncoil, int ncg,
-            const unsigned *__restrict__ kq, const float *__restrict__ ksc)
+// Block step of one k-space row (row r, gn coils of L pencils of length N in
+// LDS at stride N, after the forward transform): every location's L x L
+// Hermitian block times its L-vector, per coil, in place.
+template <int N, int L, bool MASKED, int KF, int NT, int GMAX, bool MF>
+__device__ __forceinline__ void mr_xbd_ct_block(float2 *A, const float2 *__restrict__ ker,
+                                                const int *__restrict__ inv, long long kstride,
+                                                const mr_dims &d, long long r, int gn,
+                                                const unsigned *__restrict__ kq,
+                                                const float *__restrict__ ksc, int wave,
+                                                unsigned tid)
 {
     constexpr int NW = NT / 64;
-    constexpr int TPMAX = GMAX * L;
-    extern __shared__ __align__(16) unsigned char mr_smem[];
-    float2 *A = reinterpret_cast<float2 *>(mr_smem);
-    const int Nx = d.n[0], cnx = d.c_n[0];
-    const long long q = mr_xcd_block(blockIdx.x, gridDim.x);
-    const long long r = q / ncg;
-    const int c0 = (int)(q - r * ncg) * G;
-    const int gn = (ncoil - c0 < G) ? ncoil - c0 : G;
-    const int tp = gn * L;
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const mr_tw_buf twb{mr_rsrc(tws, (unsigned)N * 8u)};
-    // pencil pl = g*L + l of the workgroup is volume v = L*c0 + pl; every
-    // wavefront loads and stores whole pencils, so a pencil's base is uniform
-    const float2 *Yr = Y + (long long)Nx * r + (long long)Nx * R * L * c0;
-    const long long pstride = (long long)Nx * R;
-    const unsigned rowbytes = (unsigned)Nx * 8u;
-
-    // Only the Nx image points of a pencil are loaded (padding costs no load
-    // instructions, which are what bounds this kernel); the N - Nx padding
-    // slots e = (s - cnx) mod N, s >= Nx, are written with zeros.
-    const int nck = (Nx + 63) >> 6;
-    // two points per lane when the pencils and their slots pair up (Nx a
-    // multiple of 4, so Nx and cnx = Nx/2 are even; 16-byte aligned rows)
-    const bool px = (Nx & 3) == 0 && ((unsigned long long)Y & 15) == 0;
-    // the image fills e < N/4 and e >= 3N/4 exactly (N = 2 Nx, centre Nx/2):
-    // the transforms skip the zeros between and the cropped outputs, and the
-    // padding slots, which the first stage then never reads, stay unwritten
-    const bool trim = mr_ct_can_trim<N>() && 2 * Nx == N && 4 * cnx == N;
-    for (int pl = wave; pl < tp; pl += NW) {
-        const __amdgpu_buffer_rsrc_t rs = mr_rsrc(Yr + pstride * pl, rowbytes);
-        float2 *Ap = A + pl * N;
-        if (px)
-            mr_xbd_load_src2<N>(Ap, rs, lane, cnx, Nx, (Nx + 127) >> 7);
-        else
-            mr_xbd_load_src<N>(Ap, rs, lane, cnx, nck);
-        if (!trim)
-            for (int i = lane; i < N - Nx; i += 64) {
-                int e = Nx + i - cnx;
-                if (e >= N) e -= N;
-                if (e < 0) e += N;
-                Ap[e] = make_float2(0.f, 0.f);
-            }
-    }
-    __syncthreads();
-    if constexpr (mr_ct_can_trim<N>()) {
-        if (trim)
-            mr_ct_fft_trim<N, N, false, NT, TPMAX>(A, twb, tp, wave);
-        else
-            mr_ct_fft<N, N, 0, false, NT, TPMAX>(A, twb, tp, wave);
-    } else {
-        mr_ct_fft<N, N, 0, false, NT, TPMAX>(A, twb, tp, wave);
-    }
-
+    const int lane = (int)(tid & 63u);
     const int My = d.m[1];
     const int jy = (int)(r % My), jz = (int)(r / My);
     int ky = jy + d.c_m[1]; if (ky >= My) ky -= My;
@@ -266,7 +211,7 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
             // wavefronts load the same entries as each other, mostly from L2.
             constexpr int NS = mr_xbd_split_locs(N, NT, GMAX, MASKED);
             constexpr int NP = N - NS;
-            const int t = threadIdx.x;
+            const int t = (int)tid;
             const bool split = x2 && gn == 2;
             if (t < (split ? NP : nitem)) {
                 item(t);
@@ -287,16 +232,16 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
         } else if constexpr (((GMAX + 1) / 2) * N <= NT) {
             // a thread per item: one round, no loop (a loop here keeps the
             // walk's column bases live across iterations and spills)
-            if ((int)threadIdx.x < nitem) item(threadIdx.x);
+            if ((int)tid < nitem) item((int)tid);
         } else {
-            for (int wi = threadIdx.x; wi < nitem; wi += NT) item(wi);
+            for (int wi = (int)tid; wi < nitem; wi += NT) item(wi);
         }
     } else {
     // FP32, one walk per coil: entries two per 16-byte load over lane pairs
     // under the conditions of the pair walk
     const bool x1 = KF == MR_KF_FP32 && !MASKED && N % 64 == 0 && d.c_m[0] % 2 == 0 &&
                     ((unsigned long long)ker & 15) == 0;
-    for (int wi = threadIdx.x; wi < gn * N; wi += NT) {
+    for (int wi = (int)tid; wi < gn * N; wi += NT) {
         const int g = (int)mr_cdiv<N>::q((unsigned)wi);
         int e = wi - g * N;
         const int hi = (e & 63) >> 5;
@@ -324,6 +269,78 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
         }
     }
     }
+}
+
+// NT threads, up to GMAX coils per workgroup (the default: two workgroups of
+// 512 threads per CU; the wide form: 1024 threads and two coils in 160 KB,
+// one workgroup per CU). Either way a CU holds 16 wavefronts.
+// MF: the block step on the matrix cores (mr_bd_mfma32; FP32 kernel,
+// unmasked, 16-byte aligned kernel), a wavefront per 32 locations of a coil
+// pair.
+template <int N, int L, bool MASKED, int KF, int NT = MR_XBD_THREADS,
+          int GMAX = mr_xbd_ct_gmax(N, L), bool MF = false>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
+mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__restrict__ inv,
+            long long kstride, mr_dims d, const float2 *tws, long long R, int G, int ncoil, int ncg,
+            const unsigned *__restrict__ kq, const float *__restrict__ ksc)
+{
+    constexpr int NW = NT / 64;
+    constexpr int TPMAX = GMAX * L;
+    extern __shared__ __align__(16) unsigned char mr_smem[];
+    float2 *A = reinterpret_cast<float2 *>(mr_smem);
+    const int Nx = d.n[0], cnx = d.c_n[0];
+    const long long q = mr_xcd_block(blockIdx.x, gridDim.x);
+    const long long r = q / ncg;
+    const int c0 = (int)(q - r * ncg) * G;
+    const int gn = (ncoil - c0 < G) ? ncoil - c0 : G;
+    const int tp = gn * L;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const mr_tw_buf twb{mr_rsrc(tws, (unsigned)N * 8u)};
+    // pencil pl = g*L + l of the workgroup is volume v = L*c0 + pl; every
+    // wavefront loads and stores whole pencils, so a pencil's base is uniform
+    const float2 *Yr = Y + (long long)Nx * r + (long long)Nx * R * L * c0;
+    const long long pstride = (long long)Nx * R;
+    const unsigned rowbytes = (unsigned)Nx * 8u;
+
+    // Only the Nx image points of a pencil are loaded (padding costs no load
+    // instructions, which are what bounds this kernel); the N - Nx padding
+    // slots e = (s - cnx) mod N, s >= Nx, are written with zeros.
+    const int nck = (Nx + 63) >> 6;
+    // two points per lane when the pencils and their slots pair up (Nx a
+    // multiple of 4, so Nx and cnx = Nx/2 are even; 16-byte aligned rows)
+    const bool px = (Nx & 3) == 0 && ((unsigned long long)Y & 15) == 0;
+    // the image fills e < N/4 and e >= 3N/4 exactly (N = 2 Nx, centre Nx/2):
+    // the transforms skip the zeros between and the cropped outputs, and the
+    // padding slots, which the first stage then never reads, stay unwritten
+    const bool trim = mr_ct_can_trim<N>() && 2 * Nx == N && 4 * cnx == N;
+    for (int pl = wave; pl < tp; pl += NW) {
+        const __amdgpu_buffer_rsrc_t rs = mr_rsrc(Yr + pstride * pl, rowbytes);
+        float2 *Ap = A + pl * N;
+        if (px)
+            mr_xbd_load_src2<N>(Ap, rs, lane, cnx, Nx, (Nx + 127) >> 7);
+        else
+            mr_xbd_load_src<N>(Ap, rs, lane, cnx, nck);
+        if (!trim)
+            for (int i = lane; i < N - Nx; i += 64) {
+                int e = Nx + i - cnx;
+                if (e >= N) e -= N;
+                if (e < 0) e += N;
+                Ap[e] = make_float2(0.f, 0.f);
+            }
+    }
+    __syncthreads();
+    if constexpr (mr_ct_can_trim<N>()) {
+        if (trim)
+            mr_ct_fft_trim<N, N, false, NT, TPMAX>(A, twb, tp, wave);
+        else
+            mr_ct_fft<N, N, 0, false, NT, TPMAX>(A, twb, tp, wave);
+    } else {
+        mr_ct_fft<N, N, 0, false, NT, TPMAX>(A, twb, tp, wave);
+    }
+
+    mr_xbd_ct_block<N, L, MASKED, KF, NT, GMAX, MF>(A, ker, inv, kstride, d, r, gn, kq, ksc, wave,
+                                                   threadIdx.x);
     __syncthreads();
     if constexpr (mr_ct_can_trim<N>()) {
         if (trim)
@@ -351,6 +368,132 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
                 mr_bstore(rs, (unsigned)s * 8u, Ap[e]);
             }
         }
+    }
+}
+
+// Persistent form of the two-coil 1024-thread fused pass, for a centred image
+// filling half the x grid (N = 2 Nx, centre Nx/2, Nx a multiple of 4, 16-byte
+// aligned rows, FP32 kernel). A workgroup per CU walks (row, coil-pair)
+// items, interleaved with the other workgroups of its XCD, and keeps each item's
+// image half a pencil on, in slots [N/4, 3N/4). The last inverse stage reads
+// every slot and then writes only the kept ones, [0, N/4) and [3N/4, N), so
+// from that point the next item's image rows stream straight from HBM into
+// the middle slots (global_load_lds_dwordx4, no registers) while this item's
+// last stage and its stores run: the load latency that a workgroup of the
+// one-item form waits out before its first transform is hidden. The LDS
+// loads are written in asm so that the compiler, which does not see them,
+// adds no wait for them to the LDS reads of the kept slots; the kernel waits
+// for them itself (vmcnt counts them with the stores issued after them, and
+// completes in order) before the next item's transform reads them.
+template <int N, int L, bool MASKED>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
+mr_k_xbd_pd(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *__restrict__ inv,
+            long long kstride, mr_dims d, const float2 *tws, long long R, int G, int ncoil, int ncg,
+            const unsigned *__restrict__ kq, const float *__restrict__ ksc)
+{
+    constexpr int NT = 1024, NW = NT / 64, GMAX = 2, TPMAX = GMAX * L;
+    constexpr int NX = N / 2, NQ = N / 4;
+    constexpr int SPP = (NX + 127) / 128;          // 16-byte stores per pencil and wavefront
+    constexpr int NBLK = (NX * 8 + 1023) / 1024;   // 1 KB LDS loads per pencil
+    static_assert(mr_ct_can_trim<N>() && NX % 4 == 0 && 2 * SPP <= 63, "persistent fused form");
+    extern __shared__ __align__(16) unsigned char mr_smem[];
+    float2 *A = reinterpret_cast<float2 *>(mr_smem);
+    const unsigned lds0 = (unsigned)(size_t)A;
+    // lane index read afresh where used (see tid below)
+    auto lane_now = [] {
+        unsigned t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+        return (int)(t & 63u);
+    };
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const mr_tw_buf twb{mr_rsrc(tws, (unsigned)N * 8u)};
+    const long long pstride = (long long)NX * R;
+    // XCD x (blockIdx.x mod 8) owns a contiguous share [qa, qb) of the items,
+    // in proportion to its workgroups; its cx workgroups take every cx-th
+    // item, so at any time they work on neighbouring items, mostly the coil
+    // groups of the same few rows, whose kernel rows its L2 then holds once
+    const int nxcd = 8, xcd = blockIdx.x % nxcd, iw = blockIdx.x / nxcd;
+    const int g8 = gridDim.x / nxcd, g8r = gridDim.x % nxcd;
+    const int cx = g8 + (xcd < g8r ? 1 : 0);
+    const long long before = (long long)xcd * g8 + (xcd < g8r ? xcd : g8r);
+    const long long nq = R * ncg;
+    const long long qa = nq * before / gridDim.x + iw, qb = nq * (before + cx) / gridDim.x;
+    if (qa >= qb) return;
+    // the image rows of item q into slots [N/4, 3N/4) of its pencils
+    auto fetch = [&](long long q) {
+        const long long r = q / ncg;
+        const int c0 = (int)(q - r * ncg) * G;
+        const int tp = ((ncoil - c0 < G) ? ncoil - c0 : G) * L;
+        const char *Yr = reinterpret_cast<const char *>(Y + NX * r + NX * R * L * c0);
+        const int lane = lane_now();
+        for (int t = wave; t < tp * NBLK; t += NW) {
+            const int pl = t / NBLK, b = t - pl * NBLK;
+            const unsigned off = (unsigned)(b * 1024 + lane * 16);
+            if (off < NX * 8u) {
+                const char *src = Yr + pstride * pl * 8 + off;
+                const unsigned m0 = __builtin_amdgcn_readfirstlane(
+                    lds0 + (unsigned)(pl * N + NQ) * 8u + (unsigned)b * 1024u);
+                // m0 is listed as written (nothing else in this kernel keeps a
+                // value in it; clang warns that it is a reserved register)
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+                asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                             :
+                             : "s"(m0), "v"(src)
+                             : "memory", "m0");
+#pragma clang diagnostic pop
+            }
+        }
+    };
+    fetch(qa);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    mr_lds_barrier();
+    for (long long q = qa; q < qb; q += cx) {
+        const long long r = q / ncg;
+        const int c0 = (int)(q - r * ncg) * G;
+        const int gn = (ncoil - c0 < G) ? ncoil - c0 : G;
+        const int tp = gn * L;
+        mr_ct_fft_mid_fwd<N, N, NT, TPMAX>(A, twb, tp, wave);
+        // the thread index is re-read for every item (an empty asm hides that
+        // it repeats), so what derives from it is recomputed, not held in
+        // registers across the loop
+        unsigned tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        mr_xbd_ct_block<N, L, MASKED, MR_KF_FP32, NT, GMAX, false>(A, ker, inv, kstride, d, r, gn,
+                                                                  kq, ksc, wave, tid);
+        mr_lds_barrier();
+        const bool more = q + cx < qb;
+        auto next = [&] {
+            if (more) fetch(q + cx);
+        };
+        mr_ct_fft_crop_inv<N, N, NT, TPMAX>(A, twb, tp, wave, mr_ct_mid<decltype(next)>{next});
+        const float2 *Yr = Y + NX * r + NX * R * L * c0;
+        const int lane = lane_now();
+        for (int pl = wave; pl < tp; pl += NW) {
+            const __amdgpu_buffer_rsrc_t rs = mr_rsrc(Yr + pstride * pl, (unsigned)NX * 8u);
+            const float2 *Ap = A + pl * N;
+#pragma unroll
+            for (int i = 0; i < SPP; ++i) {
+                const int s = 2 * lane + 128 * i;
+                int e = s - NQ;
+                if (e < 0) e += N;
+                if (s < NX)
+                    __builtin_amdgcn_raw_buffer_store_b128(
+                        *reinterpret_cast<const mr_u32x4_ld *>(Ap + e), rs, s * 8, 0, 0);
+            }
+        }
+        // the next item's rows have landed once at most this wavefront's own
+        // stores, issued after them, are outstanding
+        if (tp == 2 * NW) {
+            if constexpr (SPP == 3) __builtin_amdgcn_s_waitcnt(0x0F76);
+            else __builtin_amdgcn_s_waitcnt(0x0F70);
+        } else if (tp == NW) {
+            if constexpr (SPP == 3) __builtin_amdgcn_s_waitcnt(0x0F73);
+            else __builtin_amdgcn_s_waitcnt(0x0F70);
+        } else {
+            __builtin_amdgcn_s_waitcnt(0x0F70);
+        }
+        mr_lds_barrier();
     }
 }
 
@@ -449,6 +592,14 @@ mr_xbd_ct_fn mr_xbd_ct_pick_wide(int n, int L, bool masked, int kf, bool mf)
                       : mr_k_xbd_ct<640, 16, false, MR_KF_FP32, 1024, 2>;
     return masked ? mr_k_xbd_ct<512, 16, true, MR_KF_FP32, 1024, 2>
                   : mr_k_xbd_ct<512, 16, false, MR_KF_FP32, 1024, 2>;
+}
+
+mr_xbd_ct_fn mr_xbd_ct_pick_persistent(int n, int L, bool masked, int kf)
+{
+    if (L != 16 || kf != MR_KF_FP32) return nullptr;
+    if (n == 640) return masked ? mr_k_xbd_pd<640, 16, true> : mr_k_xbd_pd<640, 16, false>;
+    if (n == 512) return masked ? mr_k_xbd_pd<512, 16, true> : mr_k_xbd_pd<512, 16, false>;
+    return nullptr;
 }
 
 mr_xbd_ct_fn mr_xbd_ct_pick(int n, int L, bool masked, int kf, int *gmax, bool mf)
