@@ -536,6 +536,7 @@ def main():
 
     group = info["group"][0]
     xbd = int(_lib.lib.mr_op_xbd_group(op._handle))
+    xbd_form = int(_lib.lib.mr_op_xbd_form(op._handle))
     t_in_use = op.tiles
     sb = stage_bytes(cfg, group, xbd, "fp32")
     stages = {}
@@ -548,7 +549,8 @@ def main():
     dom = max((s for s in stages if s in sb), key=lambda s: stages[s]["avg_ms"])
     ct_axis = {256, 320, 384, 512, 640, 56}
     if dom == "BD":
-        dom_kernel = "mr_k_xbd_ct" if xbd else ("mr_k_bd" if cfg.L <= 16 else "mr_k_bdt")
+        dom_kernel = ({2: "mr_k_xbd_pd"}.get(xbd_form, "mr_k_xbd_ct") if xbd
+                      else ("mr_k_bd" if cfg.L <= 16 else "mr_k_bdt"))
     else:
         dom_kernel = "mr_fft_axis_ct" if set(cfg.phi2_dim[1:]) <= ct_axis else "mr_fft_axis_fast"
     dom_ms = stages[dom]["avg_ms"]

@@ -35,6 +35,15 @@ MR_EXPORT int mr_op_profile(mr_op op, double *ms, int *calls, int n);
 MR_EXPORT int mr_aha_plan_xbd_group(mr_aha_plan plan);
 MR_EXPORT int mr_op_xbd_group(mr_op op);
 
+/* Form of the fused x pass a plan launches: 0 = separate passes (no fused
+ * pass), 1 = one workgroup per (k-space row, coil group), 2 = the persistent
+ * form (one workgroup per CU walking its items, the next row's image loaded
+ * into LDS while the current one finishes), 3 = the block step on the matrix
+ * cores (for a 16-byte aligned kernel). Negative on error. No counterpart in
+ * the reference. */
+MR_EXPORT int mr_aha_plan_xbd_form(mr_aha_plan plan);
+MR_EXPORT int mr_op_xbd_form(mr_op op);
+
 /* Autotuner of the fused route (no counterpart in the reference, whose cuFFT
  * plans fix their own tiling). For each axis pass of the apply (outer-axis
  * FFT, y-FFT, inverse y-FFT, inverse outer-axis FFT; 2D: the first and last)

@@ -190,6 +190,8 @@ _decl("mr_op_profile", ctypes.c_int, [_p, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_int), ctypes.c_int])
 _decl("mr_aha_plan_xbd_group", ctypes.c_int, [_p])
 _decl("mr_op_xbd_group", ctypes.c_int, [_p])
+_decl("mr_aha_plan_xbd_form", ctypes.c_int, [_p])
+_decl("mr_op_xbd_form", ctypes.c_int, [_p])
 _decl("mr_aha_plan_autotune", ctypes.c_int, [_p, _p, _p, _p, _p, _p, ctypes.c_int])
 _decl("mr_op_autotune", ctypes.c_int, [_p, _p, _p, ctypes.c_int])
 _decl("mr_aha_plan_tiles", ctypes.c_int, [_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int])

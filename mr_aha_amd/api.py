@@ -481,6 +481,13 @@ class AhAOp:
         No counterpart in the reference."""
         return int(lib.mr_op_xbd_group(self._open()))
 
+    @property
+    def xbd_form(self) -> int:
+        """Form of the fused x pass: 0 none, 1 one workgroup per row and coil
+        group, 2 persistent, 3 block step on the matrix cores. No counterpart
+        in the reference."""
+        return int(lib.mr_op_xbd_form(self._open()))
+
     def set_kernel_format(self, fmt: str) -> None:
         """Kernel storage read by the applies: "fp32", "bf16" or "fp16"
         (library entry mr_op_set_kernel_format; 16-bit entries with one FP32

@@ -870,6 +870,15 @@ extern "C" MR_EXPORT int mr_aha_plan_xbd_group(mr_aha_plan p)
     MR_REQUIRE(p != nullptr, "mr_aha_plan_xbd_group: plan is NULL");
     return p->xbd_g;
 }
+extern "C" MR_EXPORT int mr_aha_plan_xbd_form(mr_aha_plan p)
+{
+    mr_clear_error();
+    MR_REQUIRE(p != nullptr, "mr_aha_plan_xbd_form: plan is NULL");
+    if (p->xbd_g == 0) return 0;
+    if (p->xbd_ct && p->xbd_mf && !p->no_mf) return 3;
+    if (p->xbd_ct && p->xbd_pd) return 2;
+    return 1;
+}
 
 extern "C" MR_EXPORT int mr_aha_plan_create(const mr_config *cfg, mr_aha_plan *plan_out)
 {

@@ -469,3 +469,11 @@ extern "C" MR_EXPORT int mr_op_xbd_group(mr_op op)
     if (!op->plan) return 0;
     return mr_aha_plan_xbd_group(op->plan);
 }
+
+extern "C" MR_EXPORT int mr_op_xbd_form(mr_op op)
+{
+    mr_clear_error();
+    MR_REQUIRE(op != nullptr, "mr_op_xbd_form: op is NULL");
+    if (!op->plan) return 0;
+    return mr_aha_plan_xbd_form(op->plan);
+}

@@ -26,10 +26,11 @@ def declared_symbols():
 
 def test_exports_every_declared_symbol(L):
     syms = declared_symbols()
-    # 25 reference entry points + mr_version + 12 extensions (profiling,
-    # pipeline and tiling queries, autotuner, kernel format, CG and its
-    # allocation-free variant, the streaming copy of the bandwidth calibration)
-    assert len(syms) == 38
+    # 25 reference entry points + mr_version + 14 extensions (profiling,
+    # pipeline, fused-form and tiling queries, autotuner, kernel format, CG and
+    # its allocation-free variant, the streaming copy of the bandwidth
+    # calibration)
+    assert len(syms) == 40
     for s in syms:
         assert hasattr(L.lib, s), s
     out = subprocess.run(["nm", "-D", "--defined-only", str(L.LIB_PATH)], capture_output=True,

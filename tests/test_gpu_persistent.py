@@ -40,6 +40,18 @@ def host(t):
     return t.detach().cpu().numpy()
 
 
+def xbd_form(cfg):
+    """Fused-pass form a plan for cfg launches (mr_aha_plan_xbd_form)."""
+    import ctypes
+    from mr_aha_amd import _lib
+    plan = ctypes.c_void_p()
+    _lib.check(_lib.lib.mr_aha_plan_create(ctypes.byref(cfg), ctypes.byref(plan)), "plan_create")
+    try:
+        return int(_lib.lib.mr_aha_plan_xbd_form(plan))
+    finally:
+        _lib.lib.mr_aha_plan_destroy(plan)
+
+
 # (image n, coils): x grid 640 and 512 (grid 2n x 2n x nz); rows x coil
 # groups = 1024 and 512 items over the workgroups, one per CU
 SHAPES = [((320, 32, 8), 3), ((256, 32, 8), 2)]
@@ -76,6 +88,7 @@ def test_persistent_fused_pass(mr, monkeypatch, n, C, masked):
         outs = {}
         for pd in ("1", "0"):
             monkeypatch.setenv("MR_AHA_HIP_XBD_PD", pd)
+            assert xbd_form(cfg) == (2 if pd == "1" else 1)
             outs[pd] = host(mr.aha_toeplitz(cfg, dx, dS, dK, dmask))
         d = rel(outs["1"], outs["0"])
         print(f"[persistent]   {tri}: persistent vs one-item form {d:.2e}", flush=True)
