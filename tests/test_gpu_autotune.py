@@ -2,8 +2,10 @@
 counterpart in the reference): it keeps a tiling from the candidates it times,
 the applies after it agree with the applies before it (the tile width changes
 only which pencils share a workgroup, not the arithmetic of any pencil, so
-equality is bit for bit), and its argument checks return the documented
-errors."""
+equality is bit for bit; when it trades the persistent two-coil form of the
+fused x pass for one coil per workgroup, another kernel runs the same
+operations and they agree to 1e-6), and its argument checks return the
+documented errors."""
 
 import ctypes
 
@@ -67,6 +69,7 @@ def test_autotune_keeps_results(mr, tt, n, L, C, masked):
     with mr.create_op(cfg, dS, None, None, _phi2_toep=dK, mask=dmask) as op:
         y0 = host(op(dx))
         assert rel(y0, ref) < 1e-5
+        form0 = op.xbd_form
         before = op.tiles
         assert (before["fo"], before["fy"], before["iy"], before["io"]) == (0, 0, 0, 0)
         after = op.autotune(dx, reps=1)
@@ -79,7 +82,13 @@ def test_autotune_keeps_results(mr, tt, n, L, C, masked):
             assert all(t in (1, 2, 4, 8, 16, 32, 64) for t in inner), after
         for _ in range(2):
             y1 = host(op(dx))
-            assert np.array_equal(y1, y0), (after, rel(y1, y0))
+            if op.xbd_form == form0:
+                assert np.array_equal(y1, y0), (after, rel(y1, y0))
+            else:
+                # the two-coil form ran persistent and the tuner kept one coil
+                # per workgroup: another kernel with the same operations, whose
+                # products the compiler may contract into FMAs differently
+                assert rel(y1, y0) < 1e-6, (after, rel(y1, y0))
 
 
 def test_autotune_errors(mr):
