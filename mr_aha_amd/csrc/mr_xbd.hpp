@@ -239,8 +239,10 @@ __device__ __forceinline__ void mr_x2_step(float2 (&y0)[L], float2 (&y1)[L], mr_
         if constexpr (j + 1 < len) {
             // lanes 0-31 hold row j of the pair, 32-63 row j+1: after the
             // swap every lane has rows j and j+1 of its own location
-            const auto sx = __builtin_amdgcn_permlane32_swap(q.x, q.z, false, false);
-            const auto sy = __builtin_amdgcn_permlane32_swap(q.y, q.w, false, false);
+            unsigned qx = q.x, qy = q.y, qz = q.z, qw = q.w;
+            asm volatile("" : "+v"(qx), "+v"(qy), "+v"(qz), "+v"(qw));
+            const auto sx = __builtin_amdgcn_permlane32_swap(qx, qz, false, false);
+            const auto sy = __builtin_amdgcn_permlane32_swap(qy, qw, false, false);
             e[0] = make_float2(__uint_as_float(sx[0]), __uint_as_float(sy[0]));
             e[1] = make_float2(__uint_as_float(sx[1]), __uint_as_float(sy[1]));
         } else {
