@@ -10,6 +10,8 @@
 
 #include <type_traits>
 
+typedef unsigned int mr_u32x4_ct __attribute__((ext_vector_type(4)));
+
 // Stage radices of n by the rule of mr_fft_factor (8s, then 10, 6, 4, 2, 3, 5, 7)
 // and the product Ns of the radices before each stage.
 struct mr_ct_fac {
@@ -133,11 +135,20 @@ __device__ __forceinline__ unsigned mr_ct_swz(unsigned p) { return (p & 7u) << 2
 // last inverse stage when only outputs e < N/4 and e >= 3N/4 are kept (the
 // crop of the same pencil): outputs landing wholly between are not stored,
 // and their arithmetic is dropped with them.
+// SZ = 1: a first stage (NS = 1, radix 8) stores output P of a pencil at
+// P ^ 2((P >> 4) & 3); SZ = 2: the second stage reads its inputs there.
+// A first-stage lane writes its 8 outputs as 64 contiguous bytes, so in plain
+// order the 16-byte writes of 8 neighbouring lanes hit two of the eight
+// 16-byte bank groups (4-way conflicts); the swap of 16-byte units within
+// each 64-byte block by bits 4-5 of the position spreads them over all eight.
+// The second stage's reads stay permutations within aligned 16-element runs.
 template <int N, int NSTR, int R, int NS, bool INV, int NT, int TPMAX, class TW, bool SWI = false,
-          bool SWO = false, int PR = 0, bool LB = false, class MID = mr_ct_nomid>
+          bool SWO = false, int PR = 0, bool LB = false, class MID = mr_ct_nomid, int SZ = 0>
 __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int wave,
                                             const MID &mid = MID{})
 {
+    static_assert(SZ != 1 || (NS == 1 && R == 8 && !SWO && NSTR % 2 == 0), "swizzled first stage");
+    static_assert(SZ != 2 || ((N / R) % 16 == 0 && !SWI && PR == 0), "swizzled second stage");
     static_assert((PR != 1 && PR != 4) || (R == 8 && NS == 1 && !INV && !SWI),
                   "padded first stage");
     static_assert((PR != 2 && PR != 3) || (NS * R == N && INV && !SWO && N % 4 == 0),
@@ -179,6 +190,13 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                 const unsigned c = mr_ct_swz(p);
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[k][r] = A[p * NSTR + ((j + r * m) ^ c)];
+            } else if constexpr (SZ == 2) {
+                // position j + r m, with m a multiple of 16: bits 4-5 are
+                // those of (j >> 4) + r m / 16
+                const unsigned jh = j >> 4;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    v[k][r] = A[p * NSTR + r * m + (j ^ (2u * ((jh + (unsigned)(r * (m / 16))) & 3u)))];
             } else if constexpr (PR == 1) {
                 v[k][0] = a[0];
                 v[k][1] = a[m];
@@ -264,6 +282,19 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                     const unsigned c = mr_ct_swz(p), o = jn * (NS * R) + kk;
 #pragma unroll
                     for (int q = 0; q < R; ++q) A[p * NSTR + ((o + q * NS) ^ c)] = v[k][q];
+                } else if constexpr (SZ == 1) {
+                    // outputs 8j .. 8j + 7 (NS = 1: jn = j, kk = 0), 16-byte
+                    // pairs at 8j + (2c ^ f), f = 2 ((j >> 1) & 3)
+                    const unsigned f = 2u * ((j >> 1) & 3u);
+                    float2 *row = A + p * NSTR + 8 * j;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const mr_u32x4_ct w4 = {__float_as_uint(v[k][2 * c].x),
+                                                __float_as_uint(v[k][2 * c].y),
+                                                __float_as_uint(v[k][2 * c + 1].x),
+                                                __float_as_uint(v[k][2 * c + 1].y)};
+                        *reinterpret_cast<mr_u32x4_ct *>(row + ((2u * c) ^ f)) = w4;
+                    }
                 } else {
 #pragma unroll
                     for (int q = 0; q < R; ++q) {
@@ -288,17 +319,26 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
 
 // Stages S .. SE - 1 of the transform (SE = 99: to the last); LB as in
 // mr_ct_stage.
+// Z: the first two stages exchange data in the bank-spread order of
+// mr_ct_stage's SZ (where mr_ct_can_swz allows it).
+template <int N> constexpr bool mr_ct_can_swz()
+{
+    return mr_ct_factor(N).nst >= 2 && mr_ct_factor(N).r[0] == 8 &&
+           (N / mr_ct_factor(N).r[1]) % 16 == 0 && N % 2 == 0;
+}
 template <int N, int NSTR, int S, bool INV, int NT, int TPMAX, class TW, bool SW = false, int SE = 99,
-          bool LB = false>
+          bool LB = false, bool Z = false>
 __device__ __forceinline__ void mr_ct_fft(float2 *A, const TW &tw, int tp, int wave)
 {
     constexpr mr_ct_fac F = mr_ct_factor(N);
     static_assert(F.rest == 1, "length must be 2^a 3^b 5^c 7^d");
     static_assert(!SW || (N % 32 == 0 && F.nst >= 2), "swizzled order needs 32 | N, 2 stages");
+    static_assert(!Z || (mr_ct_can_swz<N>() && !SW), "bank-spread first stages");
     if constexpr (S < F.nst && S < SE) {
         mr_ct_stage<N, NSTR, F.r[S], F.ns[S], INV, NT, TPMAX, TW, SW && S == 0,
-                    SW && S == F.nst - 1, 0, LB>(A, tw, tp, wave);
-        mr_ct_fft<N, NSTR, S + 1, INV, NT, TPMAX, TW, SW, SE, LB>(A, tw, tp, wave);
+                    SW && S == F.nst - 1, 0, LB, mr_ct_nomid, Z ? (S == 0 ? 1 : S == 1 ? 2 : 0) : 0>(
+            A, tw, tp, wave);
+        mr_ct_fft<N, NSTR, S + 1, INV, NT, TPMAX, TW, SW, SE, LB, Z>(A, tw, tp, wave);
     }
 }
 
@@ -313,13 +353,15 @@ template <int N, int NSTR, bool INV, int NT, int TPMAX, class TW>
 __device__ __forceinline__ void mr_ct_fft_trim(float2 *A, const TW &tw, int tp, int wave)
 {
     constexpr mr_ct_fac F = mr_ct_factor(N);
+    constexpr bool Z = mr_ct_can_swz<N>() && F.nst >= 3;
     static_assert(mr_ct_can_trim<N>(), "trimmed transform");
     constexpr int L = F.nst - 1;
     if constexpr (!INV) {
-        mr_ct_stage<N, NSTR, F.r[0], F.ns[0], false, NT, TPMAX, TW, false, false, 1>(A, tw, tp, wave);
-        mr_ct_fft<N, NSTR, 1, false, NT, TPMAX, TW>(A, tw, tp, wave);
+        mr_ct_stage<N, NSTR, F.r[0], F.ns[0], false, NT, TPMAX, TW, false, false, 1, false,
+                    mr_ct_nomid, Z ? 1 : 0>(A, tw, tp, wave);
+        mr_ct_fft<N, NSTR, 1, false, NT, TPMAX, TW, false, 99, false, Z>(A, tw, tp, wave);
     } else {
-        mr_ct_fft<N, NSTR, 0, true, NT, TPMAX, TW, false, L>(A, tw, tp, wave);
+        mr_ct_fft<N, NSTR, 0, true, NT, TPMAX, TW, false, L, false, Z>(A, tw, tp, wave);
         mr_ct_stage<N, NSTR, F.r[L], F.ns[L], true, NT, TPMAX, TW, false, false, 2>(A, tw, tp, wave);
     }
 }
@@ -334,18 +376,21 @@ template <int N, int NSTR, int NT, int TPMAX, class TW>
 __device__ __forceinline__ void mr_ct_fft_mid_fwd(float2 *A, const TW &tw, int tp, int wave)
 {
     constexpr mr_ct_fac F = mr_ct_factor(N);
+    constexpr bool Z = mr_ct_can_swz<N>() && F.nst >= 3;
     static_assert(mr_ct_can_trim<N>(), "trimmed transform");
-    mr_ct_stage<N, NSTR, F.r[0], F.ns[0], false, NT, TPMAX, TW, false, false, 4, true>(A, tw, tp, wave);
-    mr_ct_fft<N, NSTR, 1, false, NT, TPMAX, TW, false, 99, true>(A, tw, tp, wave);
+    mr_ct_stage<N, NSTR, F.r[0], F.ns[0], false, NT, TPMAX, TW, false, false, 4, true, mr_ct_nomid,
+                Z ? 1 : 0>(A, tw, tp, wave);
+    mr_ct_fft<N, NSTR, 1, false, NT, TPMAX, TW, false, 99, true, Z>(A, tw, tp, wave);
 }
 template <int N, int NSTR, int NT, int TPMAX, class TW, class MID>
 __device__ __forceinline__ void mr_ct_fft_crop_inv(float2 *A, const TW &tw, int tp, int wave,
                                                    const MID &mid)
 {
     constexpr mr_ct_fac F = mr_ct_factor(N);
+    constexpr bool Z = mr_ct_can_swz<N>() && F.nst >= 3;
     static_assert(mr_ct_can_trim<N>(), "trimmed transform");
     constexpr int L = F.nst - 1;
-    mr_ct_fft<N, NSTR, 0, true, NT, TPMAX, TW, false, L, true>(A, tw, tp, wave);
+    mr_ct_fft<N, NSTR, 0, true, NT, TPMAX, TW, false, L, true, Z>(A, tw, tp, wave);
     mr_ct_stage<N, NSTR, F.r[L], F.ns[L], true, NT, TPMAX, TW, false, false, 3, true, MID>(
         A, tw, tp, wave, mid);
 }
