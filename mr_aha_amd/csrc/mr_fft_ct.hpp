@@ -135,20 +135,26 @@ __device__ __forceinline__ unsigned mr_ct_swz(unsigned p) { return (p & 7u) << 2
 // last inverse stage when only outputs e < N/4 and e >= 3N/4 are kept (the
 // crop of the same pencil): outputs landing wholly between are not stored,
 // and their arithmetic is dropped with them.
-// SZ = 1: a first stage (NS = 1, radix 8) stores output P of a pencil at
-// P ^ 2((P >> 4) & 3); SZ = 2: the second stage reads its inputs there.
+// SZ bit 1: a first stage (NS = 1, radix 8) stores output P of a pencil at
+// P ^ 2((P >> 4) & 3); bit 2: the second stage reads its inputs there.
 // A first-stage lane writes its 8 outputs as 64 contiguous bytes, so in plain
 // order the 16-byte writes of 8 neighbouring lanes hit two of the eight
 // 16-byte bank groups (4-way conflicts); the swap of 16-byte units within
 // each 64-byte block by bits 4-5 of the position spreads them over all eight.
 // The second stage's reads stay permutations within aligned 16-element runs.
+// SZ bit 4: a second stage with NS = 8 and radix 8 stores output P at
+// P ^ 8((P >> 6) & 1); bit 8: the third stage (N / R = 64) reads there. The
+// second stage's writes from lanes 8 apart (positions 64 apart) otherwise
+// share banks; the flip of bit 3 by bit 6 separates them.
 template <int N, int NSTR, int R, int NS, bool INV, int NT, int TPMAX, class TW, bool SWI = false,
           bool SWO = false, int PR = 0, bool LB = false, class MID = mr_ct_nomid, int SZ = 0>
 __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int wave,
                                             const MID &mid = MID{})
 {
-    static_assert(SZ != 1 || (NS == 1 && R == 8 && !SWO && NSTR % 2 == 0), "swizzled first stage");
-    static_assert(SZ != 2 || ((N / R) % 16 == 0 && !SWI && PR == 0), "swizzled second stage");
+    static_assert(!(SZ & 1) || (NS == 1 && R == 8 && !SWO && NSTR % 2 == 0), "swizzled first stage");
+    static_assert(!(SZ & 2) || ((N / R) % 16 == 0 && !SWI && PR == 0), "swizzled second stage");
+    static_assert(!(SZ & 4) || (NS == 8 && NS * R == 64 && !SWO && PR == 0), "swizzled stage-2 store");
+    static_assert(!(SZ & 8) || (N / R == 64 && !SWI && PR != 1 && PR != 4), "swizzled stage-3 load");
     static_assert((PR != 1 && PR != 4) || (R == 8 && NS == 1 && !INV && !SWI),
                   "padded first stage");
     static_assert((PR != 2 && PR != 3) || (NS * R == N && INV && !SWO && N % 4 == 0),
@@ -190,7 +196,7 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                 const unsigned c = mr_ct_swz(p);
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[k][r] = A[p * NSTR + ((j + r * m) ^ c)];
-            } else if constexpr (SZ == 2) {
+            } else if constexpr ((SZ & 2) != 0) {
                 // position j + r m, with m a multiple of 16: bits 4-5 are
                 // those of (j >> 4) + r m / 16
                 const unsigned jh = j >> 4;
@@ -209,6 +215,10 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                 v[k][1] = a[5 * m];
                 v[k][6] = a[2 * m];
                 v[k][7] = a[3 * m];
+            } else if constexpr ((SZ & 8) != 0) {
+                // position j + 64 r, j < 64: bit 6 is r's bit 0
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[k][r] = A[p * NSTR + r * m + (j ^ (8u * (r & 1)))];
             } else {
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[k][r] = a[r * m];
@@ -282,7 +292,14 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                     const unsigned c = mr_ct_swz(p), o = jn * (NS * R) + kk;
 #pragma unroll
                     for (int q = 0; q < R; ++q) A[p * NSTR + ((o + q * NS) ^ c)] = v[k][q];
-                } else if constexpr (SZ == 1) {
+                } else if constexpr ((SZ & 4) != 0) {
+                    // output q at 64 jn + kk + 8 (q ^ s), s = jn & 1 (bit 6)
+                    const unsigned sft = 8u * (jn & 1u);
+                    float2 *be = A + p * NSTR + jn * 64 + kk + sft;
+                    float2 *bo = A + p * NSTR + jn * 64 + kk - sft;
+#pragma unroll
+                    for (int q = 0; q < R; ++q) ((q & 1) ? bo : be)[q * 8] = v[k][q];
+                } else if constexpr ((SZ & 1) != 0) {
                     // outputs 8j .. 8j + 7 (NS = 1: jn = j, kk = 0), 16-byte
                     // pairs at 8j + (2c ^ f), f = 2 ((j >> 1) & 3)
                     const unsigned f = 2u * ((j >> 1) & 3u);
@@ -326,6 +343,22 @@ template <int N> constexpr bool mr_ct_can_swz()
     return mr_ct_factor(N).nst >= 2 && mr_ct_factor(N).r[0] == 8 &&
            (N / mr_ct_factor(N).r[1]) % 16 == 0 && N % 2 == 0;
 }
+// The second and third stages exchange data in the bank-spread order of
+// SZ bits 4 and 8: factors 8, 8, R with N / R = 64.
+template <int N> constexpr bool mr_ct_can_swz2()
+{
+    return mr_ct_factor(N).nst == 3 && mr_ct_factor(N).r[0] == 8 && mr_ct_factor(N).r[1] == 8 &&
+           N / mr_ct_factor(N).r[2] == 64;
+}
+// SZ of stage S when the transform uses the bank-spread orders (Z).
+template <int N, bool Z> constexpr int mr_ct_sz(int S)
+{
+    if (!Z) return 0;
+    int sz = 0;
+    if (mr_ct_can_swz<N>()) sz |= (S == 0 ? 1 : S == 1 ? 2 : 0);
+    if (mr_ct_can_swz2<N>()) sz |= (S == 1 ? 4 : S == 2 ? 8 : 0);
+    return sz;
+}
 template <int N, int NSTR, int S, bool INV, int NT, int TPMAX, class TW, bool SW = false, int SE = 99,
           bool LB = false, bool Z = false>
 __device__ __forceinline__ void mr_ct_fft(float2 *A, const TW &tw, int tp, int wave)
@@ -333,11 +366,10 @@ __device__ __forceinline__ void mr_ct_fft(float2 *A, const TW &tw, int tp, int w
     constexpr mr_ct_fac F = mr_ct_factor(N);
     static_assert(F.rest == 1, "length must be 2^a 3^b 5^c 7^d");
     static_assert(!SW || (N % 32 == 0 && F.nst >= 2), "swizzled order needs 32 | N, 2 stages");
-    static_assert(!Z || (mr_ct_can_swz<N>() && !SW), "bank-spread first stages");
+    static_assert(!Z || !SW, "bank-spread orders with the wide-tile swizzle");
     if constexpr (S < F.nst && S < SE) {
         mr_ct_stage<N, NSTR, F.r[S], F.ns[S], INV, NT, TPMAX, TW, SW && S == 0,
-                    SW && S == F.nst - 1, 0, LB, mr_ct_nomid, Z ? (S == 0 ? 1 : S == 1 ? 2 : 0) : 0>(
-            A, tw, tp, wave);
+                    SW && S == F.nst - 1, 0, LB, mr_ct_nomid, mr_ct_sz<N, Z>(S)>(A, tw, tp, wave);
         mr_ct_fft<N, NSTR, S + 1, INV, NT, TPMAX, TW, SW, SE, LB, Z>(A, tw, tp, wave);
     }
 }
@@ -353,16 +385,17 @@ template <int N, int NSTR, bool INV, int NT, int TPMAX, class TW>
 __device__ __forceinline__ void mr_ct_fft_trim(float2 *A, const TW &tw, int tp, int wave)
 {
     constexpr mr_ct_fac F = mr_ct_factor(N);
-    constexpr bool Z = mr_ct_can_swz<N>() && F.nst >= 3;
+    constexpr bool Z = F.nst >= 3;
     static_assert(mr_ct_can_trim<N>(), "trimmed transform");
     constexpr int L = F.nst - 1;
     if constexpr (!INV) {
         mr_ct_stage<N, NSTR, F.r[0], F.ns[0], false, NT, TPMAX, TW, false, false, 1, false,
-                    mr_ct_nomid, Z ? 1 : 0>(A, tw, tp, wave);
+                    mr_ct_nomid, mr_ct_sz<N, Z>(0)>(A, tw, tp, wave);
         mr_ct_fft<N, NSTR, 1, false, NT, TPMAX, TW, false, 99, false, Z>(A, tw, tp, wave);
     } else {
         mr_ct_fft<N, NSTR, 0, true, NT, TPMAX, TW, false, L, false, Z>(A, tw, tp, wave);
-        mr_ct_stage<N, NSTR, F.r[L], F.ns[L], true, NT, TPMAX, TW, false, false, 2>(A, tw, tp, wave);
+        mr_ct_stage<N, NSTR, F.r[L], F.ns[L], true, NT, TPMAX, TW, false, false, 2, false,
+                    mr_ct_nomid, mr_ct_sz<N, Z>(L)>(A, tw, tp, wave);
     }
 }
 
@@ -376,10 +409,10 @@ template <int N, int NSTR, int NT, int TPMAX, class TW>
 __device__ __forceinline__ void mr_ct_fft_mid_fwd(float2 *A, const TW &tw, int tp, int wave)
 {
     constexpr mr_ct_fac F = mr_ct_factor(N);
-    constexpr bool Z = mr_ct_can_swz<N>() && F.nst >= 3;
+    constexpr bool Z = F.nst >= 3;
     static_assert(mr_ct_can_trim<N>(), "trimmed transform");
     mr_ct_stage<N, NSTR, F.r[0], F.ns[0], false, NT, TPMAX, TW, false, false, 4, true, mr_ct_nomid,
-                Z ? 1 : 0>(A, tw, tp, wave);
+                mr_ct_sz<N, Z>(0)>(A, tw, tp, wave);
     mr_ct_fft<N, NSTR, 1, false, NT, TPMAX, TW, false, 99, true, Z>(A, tw, tp, wave);
 }
 template <int N, int NSTR, int NT, int TPMAX, class TW, class MID>
@@ -387,11 +420,12 @@ __device__ __forceinline__ void mr_ct_fft_crop_inv(float2 *A, const TW &tw, int 
                                                    const MID &mid)
 {
     constexpr mr_ct_fac F = mr_ct_factor(N);
-    constexpr bool Z = mr_ct_can_swz<N>() && F.nst >= 3;
+    constexpr bool Z = F.nst >= 3;
     static_assert(mr_ct_can_trim<N>(), "trimmed transform");
     constexpr int L = F.nst - 1;
     mr_ct_fft<N, NSTR, 0, true, NT, TPMAX, TW, false, L, true, Z>(A, tw, tp, wave);
-    mr_ct_stage<N, NSTR, F.r[L], F.ns[L], true, NT, TPMAX, TW, false, false, 3, true, MID>(
+    mr_ct_stage<N, NSTR, F.r[L], F.ns[L], true, NT, TPMAX, TW, false, false, 3, true, MID,
+                mr_ct_sz<N, Z>(L)>(
         A, tw, tp, wave, mid);
 }
 

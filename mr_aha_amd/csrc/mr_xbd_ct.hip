@@ -334,9 +334,9 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
         if (trim)
             mr_ct_fft_trim<N, N, false, NT, TPMAX>(A, twb, tp, wave);
         else
-            mr_ct_fft<N, N, 0, false, NT, TPMAX, mr_tw_buf, false, 99, false, mr_ct_can_swz<N>()>(A, twb, tp, wave);
+            mr_ct_fft<N, N, 0, false, NT, TPMAX, mr_tw_buf, false, 99, false, true>(A, twb, tp, wave);
     } else {
-        mr_ct_fft<N, N, 0, false, NT, TPMAX, mr_tw_buf, false, 99, false, mr_ct_can_swz<N>()>(A, twb, tp, wave);
+        mr_ct_fft<N, N, 0, false, NT, TPMAX, mr_tw_buf, false, 99, false, true>(A, twb, tp, wave);
     }
 
     mr_xbd_ct_block<N, L, MASKED, KF, NT, GMAX, MF>(A, ker, inv, kstride, d, r, gn, kq, ksc, wave,
@@ -346,9 +346,9 @@ mr_k_xbd_ct(float2 *__restrict__ Y, const float2 *__restrict__ ker, const int *_
         if (trim)
             mr_ct_fft_trim<N, N, true, NT, TPMAX>(A, twb, tp, wave);
         else
-            mr_ct_fft<N, N, 0, true, NT, TPMAX, mr_tw_buf, false, 99, false, mr_ct_can_swz<N>()>(A, twb, tp, wave);
+            mr_ct_fft<N, N, 0, true, NT, TPMAX, mr_tw_buf, false, 99, false, true>(A, twb, tp, wave);
     } else {
-        mr_ct_fft<N, N, 0, true, NT, TPMAX, mr_tw_buf, false, 99, false, mr_ct_can_swz<N>()>(A, twb, tp, wave);
+        mr_ct_fft<N, N, 0, true, NT, TPMAX, mr_tw_buf, false, 99, false, true>(A, twb, tp, wave);
     }
 
     for (int pl = wave; pl < tp; pl += NW) {
