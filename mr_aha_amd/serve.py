@@ -110,6 +110,8 @@ def footprint_bytes(req: Request) -> int:
         nb = ((L + 3) // 4) * ((L + 3) // 4 + 1) // 2
         kern += (T + nb) * P * 4
     work = (n[0] * n[1] * M[2] + n[0] * M[1] * M[2]) * L * C * 8   # fused-pass buffers
+    if req.precision != "fp32":
+        work *= 2   # the 16-bit operator and its FP32 refresh operator: two plans
     vecs = 5 * N * L * 8 + N * C * 8
     return int(kern + work + build_bytes(req) + vecs)
 
@@ -189,6 +191,11 @@ class Server:
         done = threading.Semaphore(0)
 
         def finish(req):
+            # the request's torch-allocated arrays (its kernel above all) go
+            # back to the device, not to torch's cache: the library allocates
+            # its plans, 16-bit kernels and CG vectors with hipMalloc, which
+            # cannot use cached blocks, and the budget assumes they are free
+            torch.cuda.empty_cache()
             with self._lock:
                 self._used -= footprint_bytes(req)
                 self._running -= 1
