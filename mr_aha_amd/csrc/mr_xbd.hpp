@@ -307,6 +307,160 @@ __device__ __forceinline__ void mr_bd_walk_pair_x2(float2 *a0, float2 *a1, int m
     mr_x2_step<L, UPPER, D, 0, NC>(y0, y1, kq, a0, a1, ms, ker, ks, boff_pair, boff_own, vc0, vc1);
 }
 
+// mr_bd_walk_pair_x2 for the lower triangle with L even, two coils, columns
+// walked in pairs (c, c + 1): each input v_r below both is read from LDS once
+// for the two columns instead of once per column (72 instead of 136 reads per
+// coil at L = 16). The loads per column pair, in order: column c rows c, c+1
+// (16-byte pair load), column c+1 row c+1 (8-byte own-location load), then
+// for r = c+2, c+4, ..: column c rows r, r+1 and column c+1 rows r, r+1.
+// Every output accumulates its terms in the order of the column walk, so the
+// results equal mr_bd_walk_pair_x2's bit for bit.
+template <int L> struct mr_x2c_units {
+    static_assert(L % 2 == 0, "column pairs");
+    // units of the column pair starting at column c
+    static constexpr int per(int c) { return 2 + (L - c - 2); }
+    static constexpr int count()
+    {
+        int n = 0;
+        for (int c = 0; c < L; c += 2) n += per(c);
+        return n;
+    }
+    // unit u: column (c or c+1), first row, and whether it is the single load
+    struct unit { int col, row; bool single; };
+    static constexpr unit at(int u)
+    {
+        int c = 0;
+        while (u >= per(c)) u -= per(c), c += 2;
+        if (u == 0) return unit{c, c, false};
+        if (u == 1) return unit{c + 1, c + 1, true};
+        const int k = (u - 2) / 2, b = (u - 2) % 2;
+        return unit{c + b, c + 2 + 2 * k, false};
+    }
+};
+
+template <int L, int U>
+__device__ __forceinline__ mr_u32x4 mr_x2c_load(const float2 *ker, long long ks, unsigned bp,
+                                                unsigned bo)
+{
+    constexpr auto un = mr_x2c_units<L>::at(U);
+    constexpr int c = un.col, j = un.row - un.col;
+    constexpr int t0 = c * L - c * (c - 1) / 2;
+    const __amdgpu_buffer_rsrc_t rs = mr_rsrc(ker + ks * t0, 0xFFFFFFFFu);
+    if constexpr (!un.single) {
+        return __builtin_amdgcn_raw_buffer_load_b128(rs, (int)bp, (int)(ks * 8 * j), 0);
+    } else {
+        const mr_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)bo, (int)(ks * 8 * j), 0);
+        return mr_u32x4{v.x, v.y, 0u, 0u};
+    }
+}
+
+template <int L, int D, int U = 0>
+__device__ __forceinline__ void mr_x2c_prefetch(mr_u32x4 (&kq)[D], const float2 *ker, long long ks,
+                                               unsigned bp, unsigned bo)
+{
+    if constexpr (U < D && U < mr_x2c_units<L>::count()) {
+        kq[U] = mr_x2c_load<L, U>(ker, ks, bp, bo);
+        mr_x2c_prefetch<L, D, U + 1>(kq, ker, ks, bp, bo);
+    }
+}
+
+// the entries of rows r, r+1 of this lane's location from a pair load
+__device__ __forceinline__ void mr_x2_rows(const mr_u32x4 &q, float2 (&e)[2])
+{
+    unsigned qx = q.x, qy = q.y, qz = q.z, qw = q.w;
+    asm volatile("" : "+v"(qx), "+v"(qy), "+v"(qz), "+v"(qw));
+    const auto sx = __builtin_amdgcn_permlane32_swap(qx, qz, false, false);
+    const auto sy = __builtin_amdgcn_permlane32_swap(qy, qw, false, false);
+    e[0] = make_float2(__uint_as_float(sx[0]), __uint_as_float(sy[0]));
+    e[1] = make_float2(__uint_as_float(sx[1]), __uint_as_float(sy[1]));
+}
+
+// Step over unit U (the first of a pair of units, or the leading pair of a
+// column pair, which are consumed together).
+template <int L, int D, int U>
+__device__ __forceinline__ void mr_x2c_step(float2 (&y0)[L], float2 (&y1)[L], mr_u32x4 (&kq)[D],
+                                           float2 *a0, float2 *a1, int ms, const float2 *ker,
+                                           long long ks, unsigned bp, unsigned bo, float2 &vc0,
+                                           float2 &vc1, float2 &vd0, float2 &vd1)
+{
+    constexpr int NU = mr_x2c_units<L>::count();
+    if constexpr (U < NU) {
+        constexpr auto ua = mr_x2c_units<L>::at(U);
+        constexpr auto ub = mr_x2c_units<L>::at(U + 1);
+        const mr_u32x4 qa = kq[U % D], qb = kq[(U + 1) % D];
+        float2 ea[2], eb[2];
+        mr_x2_rows(qa, ea);
+        if constexpr (ub.single) eb[0] = make_float2(__uint_as_float(qb.x), __uint_as_float(qb.y));
+        else mr_x2_rows(qb, eb);
+        if constexpr (U + D < NU) kq[U % D] = mr_x2c_load<L, U + D>(ker, ks, bp, bo);
+        if constexpr (U + 1 + D < NU) kq[(U + 1) % D] = mr_x2c_load<L, U + 1 + D>(ker, ks, bp, bo);
+        if constexpr (ub.single) {
+            // column pair (c, c+1) opens: rows c, c+1 of column c, row c+1 of
+            // column c+1 (its diagonal)
+            constexpr int c = ua.col;
+            vc0 = a0[c * ms];
+            vc1 = a1[c * ms];
+            vd0 = a0[(c + 1) * ms];
+            vd1 = a1[(c + 1) * ms];
+            y0[c] = cmac(y0[c], ea[0], vc0);
+            y1[c] = cmac(y1[c], ea[0], vc1);
+            y0[c] = cmac(y0[c], ea[1], vd0);
+            y0[c + 1] = cmacc(y0[c + 1], ea[1], vc0);
+            y1[c] = cmac(y1[c], ea[1], vd1);
+            y1[c + 1] = cmacc(y1[c + 1], ea[1], vc1);
+            y0[c + 1] = cmac(y0[c + 1], eb[0], vd0);
+            y1[c + 1] = cmac(y1[c + 1], eb[0], vd1);
+            asm volatile("" : "+v"(y0[c].x), "+v"(y0[c].y), "+v"(y0[c + 1].x), "+v"(y0[c + 1].y),
+                         "+v"(y1[c].x), "+v"(y1[c].y), "+v"(y1[c + 1].x), "+v"(y1[c + 1].y)::"memory");
+        } else {
+            constexpr int c = ua.col, r = ua.row;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const float2 w0 = a0[(r + h) * ms], w1 = a1[(r + h) * ms];
+                // column c's terms, then column c+1's, as the column walk orders them
+                y0[c] = cmac(y0[c], ea[h], w0);
+                y0[r + h] = cmacc(y0[r + h], ea[h], vc0);
+                y1[c] = cmac(y1[c], ea[h], w1);
+                y1[r + h] = cmacc(y1[r + h], ea[h], vc1);
+                y0[c + 1] = cmac(y0[c + 1], eb[h], w0);
+                y0[r + h] = cmacc(y0[r + h], eb[h], vd0);
+                y1[c + 1] = cmac(y1[c + 1], eb[h], w1);
+                y1[r + h] = cmacc(y1[r + h], eb[h], vd1);
+                asm volatile("" : "+v"(y0[c].x), "+v"(y0[c].y), "+v"(y0[r + h].x), "+v"(y0[r + h].y),
+                             "+v"(y1[c].x), "+v"(y1[c].y), "+v"(y1[r + h].x), "+v"(y1[r + h].y),
+                             "+v"(y0[c + 1].x), "+v"(y0[c + 1].y), "+v"(y1[c + 1].x),
+                             "+v"(y1[c + 1].y)::"memory");
+            }
+        }
+        // the column pair is done: outputs c and c+1 are final
+        constexpr auto un = mr_x2c_units<L>::at(U + 2 < NU ? U + 2 : 0);
+        if constexpr (U + 2 >= NU || un.row == un.col) {
+            constexpr int c = ua.col;
+            a0[c * ms] = y0[c];
+            a1[c * ms] = y1[c];
+            a0[(c + 1) * ms] = y0[c + 1];
+            a1[(c + 1) * ms] = y1[c + 1];
+        }
+        mr_x2c_step<L, D, U + 2>(y0, y1, kq, a0, a1, ms, ker, ks, bp, bo, vc0, vc1, vd0, vd1);
+    }
+}
+
+template <int L, int D>
+__device__ __forceinline__ void mr_bd_walk_pair_x2c(float2 *a0, float2 *a1, int ms,
+                                                    const float2 *ker, long long ks,
+                                                    unsigned boff_pair, unsigned boff_own)
+{
+    float2 y0[L], y1[L], vc0, vc1, vd0, vd1;
+    mr_u32x4 kq[D];
+    mr_x2c_prefetch<L, D>(kq, ker, ks, boff_pair, boff_own);
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        y0[l] = make_float2(0.f, 0.f);
+        y1[l] = make_float2(0.f, 0.f);
+    }
+    mr_x2c_step<L, D, 0>(y0, y1, kq, a0, a1, ms, ker, ks, boff_pair, boff_own, vc0, vc1, vd0, vd1);
+}
+
 // ---------------------------------------------------------------------------
 // Block multiply on the matrix cores (v_mfma_f32_4x4x1_16b_f32)
 // ---------------------------------------------------------------------------

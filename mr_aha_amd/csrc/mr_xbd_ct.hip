@@ -185,6 +185,8 @@ __device__ __forceinline__ void mr_xbd_ct_block(float2 *A, const float2 *__restr
                     const unsigned bp = (unsigned)((b - hi) * 8 + (hi ? kstride * 8 : 0));
                     if (d.tri_upper)
                         mr_bd_walk_pair_x2<L, true, 4>(a0, a1, N, ker, kstride, bp, (unsigned)(b * 8));
+                    else if constexpr (L % 2 == 0)
+                        mr_bd_walk_pair_x2c<L, 4>(a0, a1, N, ker, kstride, bp, (unsigned)(b * 8));
                     else
                         mr_bd_walk_pair_x2<L, false, 4>(a0, a1, N, ker, kstride, bp, (unsigned)(b * 8));
                 } else if (d.tri_upper) {
