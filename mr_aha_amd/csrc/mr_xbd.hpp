@@ -376,8 +376,9 @@ __device__ __forceinline__ void mr_x2_rows(const mr_u32x4 &q, float2 (&e)[2])
 }
 
 // Step over unit U (the first of a pair of units, or the leading pair of a
-// column pair, which are consumed together).
-template <int L, int D, int U>
+// column pair, which are consumed together). NC = 2: coils a0 and a1 share
+// every load; NC = 1: a0 only.
+template <int L, int D, int U, int NC>
 __device__ __forceinline__ void mr_x2c_step(float2 (&y0)[L], float2 (&y1)[L], mr_u32x4 (&kq)[D],
                                            float2 *a0, float2 *a1, int ms, const float2 *ker,
                                            long long ks, unsigned bp, unsigned bo, float2 &vc0,
@@ -399,37 +400,53 @@ __device__ __forceinline__ void mr_x2c_step(float2 (&y0)[L], float2 (&y1)[L], mr
             // column c+1 (its diagonal)
             constexpr int c = ua.col;
             vc0 = a0[c * ms];
-            vc1 = a1[c * ms];
             vd0 = a0[(c + 1) * ms];
-            vd1 = a1[(c + 1) * ms];
+            if constexpr (NC == 2) {
+                vc1 = a1[c * ms];
+                vd1 = a1[(c + 1) * ms];
+            }
             y0[c] = cmac(y0[c], ea[0], vc0);
-            y1[c] = cmac(y1[c], ea[0], vc1);
+            if constexpr (NC == 2) y1[c] = cmac(y1[c], ea[0], vc1);
             y0[c] = cmac(y0[c], ea[1], vd0);
             y0[c + 1] = cmacc(y0[c + 1], ea[1], vc0);
-            y1[c] = cmac(y1[c], ea[1], vd1);
-            y1[c + 1] = cmacc(y1[c + 1], ea[1], vc1);
+            if constexpr (NC == 2) {
+                y1[c] = cmac(y1[c], ea[1], vd1);
+                y1[c + 1] = cmacc(y1[c + 1], ea[1], vc1);
+            }
             y0[c + 1] = cmac(y0[c + 1], eb[0], vd0);
-            y1[c + 1] = cmac(y1[c + 1], eb[0], vd1);
-            asm volatile("" : "+v"(y0[c].x), "+v"(y0[c].y), "+v"(y0[c + 1].x), "+v"(y0[c + 1].y),
-                         "+v"(y1[c].x), "+v"(y1[c].y), "+v"(y1[c + 1].x), "+v"(y1[c + 1].y)::"memory");
+            if constexpr (NC == 2) y1[c + 1] = cmac(y1[c + 1], eb[0], vd1);
+            if constexpr (NC == 2)
+                asm volatile("" : "+v"(y0[c].x), "+v"(y0[c].y), "+v"(y0[c + 1].x), "+v"(y0[c + 1].y),
+                             "+v"(y1[c].x), "+v"(y1[c].y), "+v"(y1[c + 1].x), "+v"(y1[c + 1].y)::"memory");
+            else
+                asm volatile("" : "+v"(y0[c].x), "+v"(y0[c].y), "+v"(y0[c + 1].x),
+                             "+v"(y0[c + 1].y)::"memory");
         } else {
             constexpr int c = ua.col, r = ua.row;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const float2 w0 = a0[(r + h) * ms], w1 = a1[(r + h) * ms];
+                const float2 w0 = a0[(r + h) * ms];
                 // column c's terms, then column c+1's, as the column walk orders them
                 y0[c] = cmac(y0[c], ea[h], w0);
                 y0[r + h] = cmacc(y0[r + h], ea[h], vc0);
-                y1[c] = cmac(y1[c], ea[h], w1);
-                y1[r + h] = cmacc(y1[r + h], ea[h], vc1);
-                y0[c + 1] = cmac(y0[c + 1], eb[h], w0);
-                y0[r + h] = cmacc(y0[r + h], eb[h], vd0);
-                y1[c + 1] = cmac(y1[c + 1], eb[h], w1);
-                y1[r + h] = cmacc(y1[r + h], eb[h], vd1);
-                asm volatile("" : "+v"(y0[c].x), "+v"(y0[c].y), "+v"(y0[r + h].x), "+v"(y0[r + h].y),
-                             "+v"(y1[c].x), "+v"(y1[c].y), "+v"(y1[r + h].x), "+v"(y1[r + h].y),
-                             "+v"(y0[c + 1].x), "+v"(y0[c + 1].y), "+v"(y1[c + 1].x),
-                             "+v"(y1[c + 1].y)::"memory");
+                if constexpr (NC == 2) {
+                    const float2 w1 = a1[(r + h) * ms];
+                    y1[c] = cmac(y1[c], ea[h], w1);
+                    y1[r + h] = cmacc(y1[r + h], ea[h], vc1);
+                    y0[c + 1] = cmac(y0[c + 1], eb[h], w0);
+                    y0[r + h] = cmacc(y0[r + h], eb[h], vd0);
+                    y1[c + 1] = cmac(y1[c + 1], eb[h], w1);
+                    y1[r + h] = cmacc(y1[r + h], eb[h], vd1);
+                    asm volatile("" : "+v"(y0[c].x), "+v"(y0[c].y), "+v"(y0[r + h].x),
+                                 "+v"(y0[r + h].y), "+v"(y1[c].x), "+v"(y1[c].y), "+v"(y1[r + h].x),
+                                 "+v"(y1[r + h].y), "+v"(y0[c + 1].x), "+v"(y0[c + 1].y),
+                                 "+v"(y1[c + 1].x), "+v"(y1[c + 1].y)::"memory");
+                } else {
+                    y0[c + 1] = cmac(y0[c + 1], eb[h], w0);
+                    y0[r + h] = cmacc(y0[r + h], eb[h], vd0);
+                    asm volatile("" : "+v"(y0[c].x), "+v"(y0[c].y), "+v"(y0[r + h].x),
+                                 "+v"(y0[r + h].y), "+v"(y0[c + 1].x), "+v"(y0[c + 1].y)::"memory");
+                }
             }
         }
         // the column pair is done: outputs c and c+1 are final
@@ -437,20 +454,22 @@ __device__ __forceinline__ void mr_x2c_step(float2 (&y0)[L], float2 (&y1)[L], mr
         if constexpr (U + 2 >= NU || un.row == un.col) {
             constexpr int c = ua.col;
             a0[c * ms] = y0[c];
-            a1[c * ms] = y1[c];
             a0[(c + 1) * ms] = y0[c + 1];
-            a1[(c + 1) * ms] = y1[c + 1];
+            if constexpr (NC == 2) {
+                a1[c * ms] = y1[c];
+                a1[(c + 1) * ms] = y1[c + 1];
+            }
         }
-        mr_x2c_step<L, D, U + 2>(y0, y1, kq, a0, a1, ms, ker, ks, bp, bo, vc0, vc1, vd0, vd1);
+        mr_x2c_step<L, D, U + 2, NC>(y0, y1, kq, a0, a1, ms, ker, ks, bp, bo, vc0, vc1, vd0, vd1);
     }
 }
 
-template <int L, int D>
+template <int L, int D, int NC = 2>
 __device__ __forceinline__ void mr_bd_walk_pair_x2c(float2 *a0, float2 *a1, int ms,
                                                     const float2 *ker, long long ks,
                                                     unsigned boff_pair, unsigned boff_own)
 {
-    float2 y0[L], y1[L], vc0, vc1, vd0, vd1;
+    float2 y0[L], y1[L], vc0, vc1 = make_float2(0.f, 0.f), vd0, vd1 = make_float2(0.f, 0.f);
     mr_u32x4 kq[D];
     mr_x2c_prefetch<L, D>(kq, ker, ks, boff_pair, boff_own);
 #pragma unroll
@@ -458,7 +477,8 @@ __device__ __forceinline__ void mr_bd_walk_pair_x2c(float2 *a0, float2 *a1, int 
         y0[l] = make_float2(0.f, 0.f);
         y1[l] = make_float2(0.f, 0.f);
     }
-    mr_x2c_step<L, D, 0>(y0, y1, kq, a0, a1, ms, ker, ks, boff_pair, boff_own, vc0, vc1, vd0, vd1);
+    mr_x2c_step<L, D, 0, NC>(y0, y1, kq, a0, a1, ms, ker, ks, boff_pair, boff_own, vc0, vc1, vd0,
+                             vd1);
 }
 
 // ---------------------------------------------------------------------------

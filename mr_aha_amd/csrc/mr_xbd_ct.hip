@@ -228,6 +228,8 @@ __device__ __forceinline__ void mr_xbd_ct_block(float2 *A, const float2 *__restr
                 const unsigned bp = (unsigned)((b - hi) * 8 + (hi ? kstride * 8 : 0));
                 if (d.tri_upper)
                     mr_bd_walk_pair_x2<L, true, 8, 1>(a, a, N, ker, kstride, bp, (unsigned)(b * 8));
+                else if constexpr (L % 2 == 0)
+                    mr_bd_walk_pair_x2c<L, 8, 1>(a, a, N, ker, kstride, bp, (unsigned)(b * 8));
                 else
                     mr_bd_walk_pair_x2<L, false, 8, 1>(a, a, N, ker, kstride, bp, (unsigned)(b * 8));
             }
@@ -257,6 +259,8 @@ __device__ __forceinline__ void mr_xbd_ct_block(float2 *A, const float2 *__restr
             const unsigned bp = (unsigned)((b - hi) * 8 + (hi ? kstride * 8 : 0));
             if (d.tri_upper)
                 mr_bd_walk_pair_x2<L, true, 8, 1>(a, a, N, ker, kstride, bp, (unsigned)(b * 8));
+            else if constexpr (L % 2 == 0)
+                mr_bd_walk_pair_x2c<L, 8, 1>(a, a, N, ker, kstride, bp, (unsigned)(b * 8));
             else
                 mr_bd_walk_pair_x2<L, false, 8, 1>(a, a, N, ker, kstride, bp, (unsigned)(b * 8));
         } else if constexpr (KF != MR_KF_FP32) {
