@@ -142,6 +142,9 @@ __device__ __forceinline__ unsigned mr_ct_swz(unsigned p) { return (p & 7u) << 2
 // 16-byte bank groups (4-way conflicts); the swap of 16-byte units within
 // each 64-byte block by bits 4-5 of the position spreads them over all eight.
 // The second stage's reads stay permutations within aligned 16-element runs.
+// SZ bits 64 / 128: the same pair of stages where the second stage's N / R is
+// a multiple of 8 but not of 16 (x length 320): P ^ 2((P >> 3) & 3), which
+// takes the first stage's writes from 4-way to 2-way conflicts.
 // SZ bit 4: a second stage with NS = 8 and radix 8 stores output P at
 // P ^ 8((P >> 6) & 1); bit 8: the third stage (N / R = 64) reads there. The
 // second stage's writes from lanes 8 apart (positions 64 apart) otherwise
@@ -155,6 +158,8 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
     static_assert(!(SZ & 2) || ((N / R) % 16 == 0 && !SWI && PR == 0), "swizzled second stage");
     static_assert(!(SZ & 4) || (NS == 8 && NS * R == 64 && !SWO && PR == 0), "swizzled stage-2 store");
     static_assert(!(SZ & 8) || (N / R == 64 && !SWI && PR != 1 && PR != 4), "swizzled stage-3 load");
+    static_assert(!(SZ & 64) || (NS == 1 && R == 8 && !SWO && NSTR % 2 == 0), "swizzled first stage");
+    static_assert(!(SZ & 128) || ((N / R) % 8 == 0 && !SWI && PR == 0), "swizzled second stage");
     static_assert((PR != 1 && PR != 4) || (R == 8 && NS == 1 && !INV && !SWI),
                   "padded first stage");
     static_assert((PR != 2 && PR != 3) || (NS * R == N && INV && !SWO && N % 4 == 0),
@@ -196,6 +201,13 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                 const unsigned c = mr_ct_swz(p);
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[k][r] = A[p * NSTR + ((j + r * m) ^ c)];
+            } else if constexpr ((SZ & 128) != 0) {
+                // position j + r m, m a multiple of 8: bits 3-4 are those of
+                // (j >> 3) + r m / 8
+                const unsigned jh = j >> 3;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    v[k][r] = A[p * NSTR + r * m + (j ^ (2u * ((jh + (unsigned)(r * (m / 8))) & 3u)))];
             } else if constexpr ((SZ & 2) != 0) {
                 // position j + r m, with m a multiple of 16: bits 4-5 are
                 // those of (j >> 4) + r m / 16
@@ -299,10 +311,11 @@ __device__ __forceinline__ void mr_ct_stage(float2 *A, const TW &tw, int tp, int
                     float2 *bo = A + p * NSTR + jn * 64 + kk - sft;
 #pragma unroll
                     for (int q = 0; q < R; ++q) ((q & 1) ? bo : be)[q * 8] = v[k][q];
-                } else if constexpr ((SZ & 1) != 0) {
+                } else if constexpr ((SZ & 65) != 0) {
                     // outputs 8j .. 8j + 7 (NS = 1: jn = j, kk = 0), 16-byte
-                    // pairs at 8j + (2c ^ f), f = 2 ((j >> 1) & 3)
-                    const unsigned f = 2u * ((j >> 1) & 3u);
+                    // pairs at 8j + (2c ^ f), f = 2 ((j >> 1) & 3) (bit 1) or
+                    // 2 (j & 3) (bit 64)
+                    const unsigned f = (SZ & 1) ? 2u * ((j >> 1) & 3u) : 2u * (j & 3u);
                     float2 *row = A + p * NSTR + 8 * j;
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
@@ -351,11 +364,17 @@ template <int N> constexpr bool mr_ct_can_swz2()
            N / mr_ct_factor(N).r[2] == 64;
 }
 // SZ of stage S when the transform uses the bank-spread orders (Z).
+template <int N> constexpr bool mr_ct_can_swz8()
+{
+    return !mr_ct_can_swz<N>() && mr_ct_factor(N).nst >= 2 && mr_ct_factor(N).r[0] == 8 &&
+           (N / mr_ct_factor(N).r[1]) % 8 == 0 && N % 2 == 0;
+}
 template <int N, bool Z> constexpr int mr_ct_sz(int S)
 {
     if (!Z) return 0;
     int sz = 0;
     if (mr_ct_can_swz<N>()) sz |= (S == 0 ? 1 : S == 1 ? 2 : 0);
+    if (mr_ct_can_swz8<N>()) sz |= (S == 0 ? 64 : S == 1 ? 128 : 0);
     if (mr_ct_can_swz2<N>()) sz |= (S == 1 ? 4 : S == 2 ? 8 : 0);
     return sz;
 }

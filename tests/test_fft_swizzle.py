@@ -29,6 +29,11 @@ def can1(n):
     return len(R) >= 2 and R[0] == 8 and (n // R[1]) % 16 == 0
 
 
+def can1b(n):
+    R = factor(n)
+    return not can1(n) and len(R) >= 2 and R[0] == 8 and (n // R[1]) % 8 == 0
+
+
 def can2(n):
     R = factor(n)
     return len(R) == 3 and R[0] == 8 and R[1] == 8 and n // R[2] == 64
@@ -42,13 +47,19 @@ def phi2(P):
     return P ^ (8 * ((P >> 6) & 1))
 
 
-def stage1_write(j, q):
+def phi1b(P):
+    return P ^ (2 * ((P >> 3) & 3))
+
+
+def stage1_write(j, q, b=False):
     """Address of first-stage output q of butterfly j (16-byte pairs)."""
-    f = 2 * ((j >> 1) & 3)
+    f = 2 * (j & 3) if b else 2 * ((j >> 1) & 3)
     return 8 * j + ((2 * (q // 2)) ^ f) + (q & 1)
 
 
-def stage2_read(j, r, m):
+def stage2_read(j, r, m, b=False):
+    if b:
+        return r * m + (j ^ (2 * (((j >> 3) + r * (m // 8)) & 3)))
     return r * m + (j ^ (2 * (((j >> 4) + r * (m // 16)) & 3)))
 
 
@@ -72,8 +83,8 @@ def stockham(x, inverse=False):
         m = n // rad
         b = np.zeros(n, np.complex128)
         for j in range(m):
-            if s == 1 and can1(n):
-                v = np.array([a[stage2_read(j, r, m)] for r in range(rad)])
+            if s == 1 and (can1(n) or can1b(n)):
+                v = np.array([a[stage2_read(j, r, m, can1b(n))] for r in range(rad)])
             elif s == 2 and can2(n):
                 v = np.array([a[stage3_read(j, r)] for r in range(rad)])
             else:
@@ -83,8 +94,8 @@ def stockham(x, inverse=False):
                 v = v * np.exp(sgn * 2j * np.pi * kk / (ns * rad)) ** np.arange(rad)
             V = np.exp(sgn * 2j * np.pi * np.outer(np.arange(rad), np.arange(rad)) / rad) @ v
             for q in range(rad):
-                if s == 0 and can1(n):
-                    b[stage1_write(j, q)] = V[q]
+                if s == 0 and (can1(n) or can1b(n)):
+                    b[stage1_write(j, q, can1b(n))] = V[q]
                 elif s == 1 and can2(n):
                     b[stage2_write(j, q)] = V[q]
                 else:
@@ -132,3 +143,24 @@ def test_addresses_are_the_layout_and_spread_banks(n):
             for q in range(8):
                 slots = {stage2_write(j, q) % 16 for j in range(j0, j0 + 16)}
                 assert len(slots) == 16
+
+
+def test_spread_order_at_320():
+    """x length 320 (8, 8, 5): the second stage's N / R = 40 admits the
+    order P ^ 2((P >> 3) & 3), which halves the first stage's conflicts."""
+    n = 320
+    R = factor(n)
+    assert can1b(n) and not can1(n)
+    m0, m1 = n // R[0], n // R[1]
+    for j in range(m0):
+        for q in range(8):
+            assert stage1_write(j, q, True) == phi1b(8 * j + q)
+    for j in range(m1):
+        for r in range(R[1]):
+            assert stage2_read(j, r, m1, True) == phi1b(j + r * m1)
+    for j0 in range(0, m0 - 7, 8):
+        for c in range(4):
+            groups = [(stage1_write(j, 2 * c, True) // 2) % 8 for j in range(j0, j0 + 8)]
+            assert len(set(groups)) == 4 and all(groups.count(g) == 2 for g in set(groups))
+            plain = [(8 * j + 2 * c) // 2 % 8 for j in range(j0, j0 + 8)]
+            assert len(set(plain)) == 2
